@@ -1,0 +1,43 @@
+/*
+ * tay_ext.h - additions to the Tay API in libtay_amd.so.  None of these has a
+ * counterpart in the reference; they expose the GPU space's telemetry and let
+ * hosts add many agents at once.
+ */
+#ifndef tay_ext_h
+#define tay_ext_h
+
+#include "tay.h"
+#include "tay_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last error set on the state ("" if none). */
+const char *tay_get_error_message(TayState *state);
+
+/* Commits the next count agents of the group's storage at once (the
+ * records at tay_get_available_agent() onwards); same as calling
+ * tay_commit_available_agent count times. */
+void tay_commit_available_agents(TayState *state, TayGroup *group, unsigned count);
+
+/* Counting of pairs that pass the box test (agent_see calls) in every see
+ * pass; takes effect at the next tay_run, which recompiles the program.  The
+ * counts cover the steps of the last tay_run. */
+void tay_set_pair_counting(TayState *state, int enable);
+int tay_get_pair_count(TayState *state, unsigned pass_index, unsigned long long *pairs);
+
+/* Device time of each pass's kernels over the last tay_run, measured with
+ * HIP events on the library stream. */
+void tay_set_pass_timing(TayState *state, int enable);
+int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, unsigned *launches);
+
+/* The device space of a group, valid while the simulation runs (NULL before
+ * the first tay_run and for groups not on the GPU). */
+TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

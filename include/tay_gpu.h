@@ -1,0 +1,152 @@
+/*
+ * tay_gpu.h - C-ABI of the MI355X GPU spaces (libtay_amd.so).
+ *
+ * These entry points replace the reference's OpenCL backend at the points
+ * where its state machine dispatches to the device (tay/state.c:209-218,
+ * 230-233, 253-254, 273-274, 296-299, 313-315; declarations in
+ * tay/ocl.h:43-96).  They take plain pointers and sizes only, so a host that
+ * keeps its own state machine can call them directly; this library's own
+ * tay.h implementation is built on exactly these calls.
+ *
+ * All device work is queued on one HIP stream owned by the library and runs
+ * in submission order.  Every call returns 1 on success and 0 on failure;
+ * tay_gpu_last_error() then describes the failure.  Agents live on the device
+ * as structure-of-arrays: one column of 32-bit words per word of the agent
+ * record.
+ */
+#ifndef tay_gpu_h
+#define tay_gpu_h
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct TayGpuSpace TayGpuSpace;
+typedef struct TayGpuProgram TayGpuProgram;
+
+typedef enum TayGpuSpaceKind {
+    TAY_GPU_KIND_Z_GRID = 1,    /* Morton-ordered uniform grid, point agents */
+    TAY_GPU_KIND_SIMPLE = 2,    /* unsorted, every seer meets every seen agent */
+} TayGpuSpaceKind;
+
+typedef enum TayGpuPassType {
+    TAY_GPU_PASS_SEE = 0,
+    TAY_GPU_PASS_ACT = 1,
+} TayGpuPassType;
+
+/* Reports which device the library uses; 0 when no GPU is present.
+ * Replaces ocl_init_context (tay/ocl_wrappers.c:213-345). */
+int tay_gpu_available(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tay_gpu_last_error(void);
+
+/* Waits for all queued device work.  Replaces ocl_finish (tay/ocl.h:94). */
+int tay_gpu_synchronize(void);
+
+/* ------------------------------------------------------------- spaces */
+
+/* Device storage for one agent group and its space.  agent_size must be a
+ * multiple of 4.  For a Z-grid, part_sizes[0..dims) are the suggested cell
+ * sizes and shared_bytes bounds the cell table (8 bytes per cell), as the
+ * reference's shared_size_in_megabytes bounds its grids
+ * (tay/state.c:109-118).  Replaces the per-group part of ocl_create_buffers
+ * (tay/ocl.c:59-98). */
+TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigned capacity, int is_point,
+                                  int dims, const float *part_sizes, unsigned long long shared_bytes);
+void tay_gpu_space_destroy(TayGpuSpace *space);
+
+/* Uses [min, max] as the grid box instead of the agents' bounding box; agents
+ * outside are clamped into the border cells.  Replaces the host-only
+ * tay_fix_space_box (tay/state.c:120-125), which the OpenCL grid ignores
+ * (todo:6). */
+int tay_gpu_space_fix_box(TayGpuSpace *space, const float *min, const float *max);
+int tay_gpu_space_unfix_box(TayGpuSpace *space);
+
+/* Copies count AoS records from host memory to the device and makes them the
+ * space's agents.  Replaces ocl_push_agents_non_blocking (tay/ocl.c:238-250). */
+int tay_gpu_space_push(TayGpuSpace *space, const void *agents, unsigned count);
+
+/* Copies the space's agents, in current storage order, to count AoS records
+ * in host memory.  Replaces ocl_fetch_agents (tay/ocl.c:261-272). */
+int tay_gpu_space_fetch(TayGpuSpace *space, void *agents, unsigned count);
+
+unsigned tay_gpu_space_count(const TayGpuSpace *space);
+
+/* Rebuilds a Z-grid from the current positions: bounding box, geometry, Morton
+ * keys, stable radix sort, cell ranges, column permutation.  No host
+ * synchronisation.  A no-op for a Simple space.  Replaces
+ * ocl_grid_run_sort_kernel (tay/ocl_grid.c:241-362); the cell reset of
+ * ocl_grid_run_unsort_kernel (tay/ocl_grid.c:364-374) has no counterpart
+ * because cell ranges are rebuilt, not accumulated. */
+int tay_gpu_space_sort(TayGpuSpace *space);
+
+/* Geometry of the last sort (synchronises).  cell_counts has 4 entries. */
+int tay_gpu_space_grid_info(TayGpuSpace *space, float *origin, float *cell_sizes, int *cell_counts,
+                            unsigned long long *cells_count);
+
+/* ------------------------------------------------------------ programs */
+
+typedef struct TayGpuPassDesc {
+    int type;                   /* TayGpuPassType */
+    const char *func_name;      /* model function to call */
+    TayGpuSpace *seer;          /* the act group for an act pass */
+    TayGpuSpace *seen;          /* see passes only */
+    float radii[4];             /* see passes only */
+    int self_see;               /* see passes only; same-space pairs (i, i) skipped when 0 */
+    unsigned context_size;      /* bytes copied by tay_gpu_program_set_context */
+} TayGpuPassDesc;
+
+/* Composes the model sources and one generated kernel per pass into a single
+ * hipRTC program for gfx950 and loads it.  On a compile error the hipRTC log
+ * goes to log (when given) and NULL is returned.  count_pairs makes every see
+ * kernel also count the pairs that pass the box test.  Replaces
+ * ocl_compile_program + ocl_get_pass_kernel (tay/ocl.c:100-177, 28-33). */
+TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsigned sources_count,
+                                      const TayGpuPassDesc *passes, unsigned passes_count,
+                                      int count_pairs, char *log, unsigned log_size);
+void tay_gpu_program_destroy(TayGpuProgram *program);
+
+/* Shape of a pass without device storage, for compiling a program where no
+ * GPU is present. */
+typedef struct TayGpuPassShape {
+    int type;                   /* TayGpuPassType */
+    const char *func_name;
+    unsigned seer_agent_size;   /* the act group's agent size for an act pass */
+    unsigned seen_agent_size;
+    int seer_is_point;
+    int seen_is_point;
+    int seen_kind;              /* TayGpuSpaceKind */
+    int dims;
+    int skip_self;              /* 1 when seer and seen are one group and self_see is 0 */
+} TayGpuPassShape;
+
+/* Composes and compiles a program to a gfx950 code object without loading
+ * it; needs no GPU.  On a compile error the hipRTC log goes to log. */
+int tay_gpu_compile_check(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
+                          unsigned shapes_count, char *log, unsigned log_size, unsigned long long *code_size);
+
+/* Uploads a pass's context (context_size bytes).  Replaces
+ * ocl_push_pass_contexts_non_blocking (tay/ocl.c:252-259). */
+int tay_gpu_program_set_context(TayGpuProgram *program, unsigned pass_index, const void *context);
+
+/* Queues one pass.  Replaces ocl_run_see_kernel / ocl_run_act_kernel
+ * (tay/ocl.c:208-236). */
+int tay_gpu_program_run_pass(TayGpuProgram *program, unsigned pass_index);
+
+/* Pairs that passed the box test (agent_see calls) since the last reset;
+ * needs count_pairs.  The reference's narrow-phase telemetry counter
+ * (tay/space.c:57-60). */
+int tay_gpu_program_pair_count(TayGpuProgram *program, unsigned pass_index, unsigned long long *pairs);
+int tay_gpu_program_reset_pair_counts(TayGpuProgram *program);
+
+/* Device time of the pass's kernels since the last reset, in milliseconds,
+ * measured with HIP events on the library stream when timing is enabled. */
+int tay_gpu_program_enable_timing(TayGpuProgram *program, int enable);
+int tay_gpu_program_pass_time(TayGpuProgram *program, unsigned pass_index, double *ms, unsigned *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

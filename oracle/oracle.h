@@ -1,0 +1,52 @@
+/*
+ * oracle.h - TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement of the reference's Tay library for the Z-grid hot path.
+ * It is the checker for the GPU library and the timed CPU baseline; nothing in
+ * the product (tay_amd/, include/) may load it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * liboracle.so exports the reference API of include/tay.h (same names and
+ * signatures) with these spaces:
+ *   TAY_CPU_SIMPLE  all pairs                (tay/cpu_simple.c)
+ *   TAY_CPU_GRID    uniform grid             (tay/cpu_grid.c)
+ *   TAY_CPU_Z_GRID  power-of-two Morton grid (tay/cpu_z_grid.c)
+ *   TAY_GPU_Z_GRID  the CPU statement of what the GPU Z-grid must compute
+ *                   (geometry of the reference's OpenCL Z-grid,
+ *                   tay/ocl_grid.c:128-136, Morton keys of tay/cpu_z_grid.c:37-83,
+ *                   stable key order, integer memory cap, x->y->z sweep)
+ * plus the inspection calls below, which have no reference counterpart.
+ */
+#ifndef tay_oracle_h
+#define tay_oracle_h
+
+#include "tay.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pairs of the pass summed over the steps of the last tay_run: candidates
+ * tested (broad) and pairs that passed the box test, i.e. agent_see calls
+ * (narrow); the reference's telemetry counters (tay/space.c:50-60,119-121). */
+void oracle_get_pair_counts(TayState *state, unsigned pass_index,
+                            unsigned long long *broad, unsigned long long *narrow);
+
+/* Geometry of a grid space after its last sort. */
+void oracle_get_grid_info(TayState *state, TayGroup *group,
+                          float4 *origin, float4 *cell_sizes, int *cell_counts /* [4] */,
+                          unsigned long long *cells_count);
+
+/* Morton key of cell indices (tay/cpu_z_grid.c:37-83): 3D 10 bits per axis,
+ * 2D 16 bits per axis, 1D identity, 4D not implemented (returns 0). */
+unsigned oracle_morton_key(int x, int y, int z, int dims);
+void oracle_morton_decode(unsigned key, int dims, int *x, int *y, int *z);
+
+/* Wall-clock seconds spent in see passes during the last tay_run. */
+double oracle_get_see_seconds(TayState *state);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

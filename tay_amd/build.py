@@ -1,0 +1,112 @@
+"""Builds libtay_amd.so (HIP, gfx950) in-tree, and the CPU oracle used by the
+tests.  Run `python -m tay_amd.build` or call build() / build_oracle().
+
+The library is compiled with the float semantics its parity depends on: no
+contraction into fused multiply-add (-ffp-contract=off), correctly rounded
+division and square root (hipcc's default), and no packed FP32 instructions.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libtay_amd.so")
+ORACLE = os.path.join(ROOT, "oracle")
+REFERENCE = "/root/reference"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + BUILD,
+]
+
+HOST_FLAGS = [
+    "-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-ffp-contract=off", "-Wall",
+    "-I" + os.path.join(ROOT, "include"),
+]
+
+SOURCES = ["gpu_space.hip", "jit.cpp", "tay_api.cpp"]
+HOST_ONLY = {"tay_api.cpp"}
+EMBEDDED = {"device_abi_text": "device_abi.h", "jit_device_text": "jit_device.h"}
+
+
+def _write_embedded():
+    """Embed device_abi.h and jit_device.h as string constants."""
+    os.makedirs(BUILD, exist_ok=True)
+    out = ['// generated by tay_amd/build.py from csrc/device_abi.h and csrc/jit_device.h',
+           '#include "internal.h"', 'namespace tay {']
+    for name, header in EMBEDDED.items():
+        with open(os.path.join(CSRC, header)) as f:
+            text = f.read()
+        if ")TAYEMBED\"" in text:
+            raise RuntimeError(header + " contains the raw-string delimiter")
+        out.append('extern const char *const %s = R"TAYEMBED(%s)TAYEMBED";' % (name, text))
+    out.append("}  // namespace tay")
+    path = os.path.join(BUILD, "embedded.cpp")
+    content = "\n".join(out) + "\n"
+    if not os.path.exists(path) or open(path).read() != content:
+        with open(path, "w") as f:
+            f.write(content)
+    return path
+
+
+def _run(cmd, cwd=None):
+    proc = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (proc.returncode, " ".join(cmd), proc.stdout))
+    return proc.stdout
+
+
+def build_library(verbose=False):
+    """Compile every HIP/C++ source for gfx950 and link libtay_amd.so."""
+    embedded = _write_embedded()
+    objs = []
+    jobs = []
+    for src in SOURCES + [embedded]:
+        path = src if os.path.isabs(src) else os.path.join(CSRC, src)
+        obj = os.path.join(BUILD, os.path.basename(path) + ".o")
+        objs.append(obj)
+        if src in HOST_ONLY:
+            # host C++ without HIP headers (sees Tay's float4, not HIP's)
+            jobs.append(["g++", "-x", "c++"] + HOST_FLAGS + ["-c", path, "-o", obj])
+        else:
+            jobs.append([HIPCC, "-x", "hip", "--offload-arch=" + ARCH] + COMMON_FLAGS + ["-c", path, "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(4, len(jobs))) as pool:
+        outs = list(pool.map(_run, jobs))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+         ["-Wl,-Bsymbolic", "-lhiprtc", "-lamdhip64"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference=True):
+    """Build the CPU oracle (test infrastructure) and, where the reference
+    tree exists, the reference's own grid code into oracle/_ref/."""
+    _run(["make", "-s", "-C", ORACLE, "all"])
+    if with_reference and os.path.isdir(os.path.join(REFERENCE, "tay")):
+        _run(["make", "-s", "-C", ORACLE, "ref", "REF=" + REFERENCE])
+
+
+def build(verbose=False):
+    lib = build_library(verbose)
+    build_oracle()
+    return lib
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

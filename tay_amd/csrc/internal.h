@@ -1,0 +1,93 @@
+/* internal.h - declarations shared by the translation units of libtay_amd. */
+#ifndef TAY_INTERNAL_H
+#define TAY_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device_abi.h"
+#include "tay_gpu.h"
+
+#define TAY_API extern "C" __attribute__((visibility("default")))
+
+namespace tay {
+
+/* Records a printf-style message as the calling thread's last error. */
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+/* Returns false (with the error recorded) when a HIP call failed. */
+bool hip_ok(hipError_t e, const char *what);
+
+/* The library's stream, created on first use on device 0.  Returns nullptr
+ * (error recorded) when no GPU can be used. */
+hipStream_t stream();
+
+/* Device allocation helpers; false with the error recorded on failure. */
+bool device_alloc(void **ptr, size_t bytes, const char *what);
+void device_free(void *ptr);
+
+constexpr unsigned kSeeBlock = 256;     /* 4 wavefronts, one 64-seer tile each */
+constexpr unsigned kActBlock = 256;
+
+/* Text of device_abi.h and jit_device.h, embedded by the build. */
+extern const char *const device_abi_text;
+extern const char *const jit_device_text;
+
+}  // namespace tay
+
+struct TayGpuSpace {
+    int kind;
+    unsigned agent_size;
+    unsigned words;             /* 32-bit words per agent */
+    unsigned capacity;
+    int is_point;
+    int dims;
+    float part[4];
+    unsigned long long shared_bytes;
+    unsigned long long max_cells;
+    unsigned count;
+
+    unsigned *cols[2];          /* structure-of-arrays agent storage, ping-pong */
+    int cur;
+
+    /* Z-grid build */
+    TayGridGeom *geom;
+    float *box_partials;
+    unsigned box_blocks;
+    unsigned *keys[2];
+    unsigned *vals[2];
+    unsigned *block_hist;
+    unsigned radix_blocks;
+    TayCellRange *ranges;
+    int box_fixed;
+    float fixed_min[4];
+    float fixed_max[4];
+
+    /* host <-> device transfers */
+    unsigned *staging;
+    size_t staging_bytes;
+};
+
+struct TayGpuProgram {
+    struct Pass {
+        TayGpuPassDesc desc;
+        std::string func_name;
+        hipFunction_t kernel;
+        void *context;              /* device copy of the pass context */
+        void *context_host;         /* pinned staging for the upload */
+        unsigned long long *pair_count;
+        /* timing: event pairs recorded around launches, resolved on query */
+        std::vector<hipEvent_t> ev_free;
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+        double ms;
+        unsigned launches;
+    };
+    hipModule_t module;
+    std::vector<Pass> passes;
+    int count_pairs;
+    int timing;
+};
+
+#endif

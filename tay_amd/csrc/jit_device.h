@@ -1,0 +1,214 @@
+/*
+ * jit_device.h - device templates for the generated see and act kernels.
+ *
+ * The build embeds this text into every hipRTC program after the model
+ * sources; each pass then instantiates one template with a call to its model
+ * function.  Agent records are rebuilt in registers from the
+ * structure-of-arrays columns, the model function runs on those private
+ * copies, and only the words it changed are stored back, so loads of columns
+ * a model never reads are removed by the compiler.
+ *
+ * See kernel (tay_see_tiles): one wavefront owns a tile of 64 consecutive
+ * seers of the sorted storage.  The tile's bounding box, grown by the radii
+ * and a rounding margin, selects the seen cells; the wavefront walks them x
+ * outermost, then y, then z, and each cell's agents in storage order, so every
+ * seer meets its pairs in the order of the reference's grid sweep
+ * (tay/cpu_grid.c:222-261) and accumulates exactly the same float sums.  The
+ * seen index is uniform across the wavefront, so seen words arrive through
+ * scalar loads and every lane applies the reference's per-axis box test
+ * (tay/space.c:62-104) to its own seer.
+ */
+
+template <typename F> struct tay_see_traits;
+template <typename A, typename B, typename C> struct tay_see_traits<void(A *, B *, C *)> {
+    typedef A seer;
+    typedef B seen;
+    typedef C context;
+};
+
+template <typename F> struct tay_act_traits;
+template <typename A, typename C> struct tay_act_traits<void(A *, C *)> {
+    typedef A agent;
+    typedef C context;
+};
+
+enum { TAY_NARROW_PP = 0, TAY_NARROW_PN = 1, TAY_NARROW_NP = 2, TAY_NARROW_NN = 3 };
+
+__device__ __forceinline__ float tay_wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fminf(v, __shfl_xor(v, o));
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+
+__device__ __forceinline__ float tay_wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = fmaxf(v, __shfl_xor(v, o));
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+
+__device__ __forceinline__ unsigned tay_wave_sum(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+/* Logical block order that hands each XCD one contiguous run of tiles: the
+ * hardware deals consecutive workgroups round-robin over the 8 XCDs, each
+ * with its own L2, and neighbouring tiles read the same seen cells. */
+__device__ __forceinline__ unsigned tay_xcd_block(void) {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned q = nb >> 3, r = nb & 7u, xcd = b & 7u, k = b >> 3;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+/* Cell index along one axis, (int)floorf((p - origin) / size) as in
+ * tay/cpu_grid.c:18-41, clamped to [0, count - 1]. */
+__device__ __forceinline__ int tay_cell_axis(float p, float origin, float size, int count) {
+    int c = (int)floorf((p - origin) / size);
+    c = c < 0 ? 0 : c;
+    return c > count - 1 ? count - 1 : c;
+}
+
+template <int NARROW, int DIMS>
+__device__ __forceinline__ bool tay_narrow(const float *a_lo, const float *a_hi, const float *b_lo, const float *b_hi,
+                                           const float *r) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < DIMS; ++k) {
+        if (NARROW == TAY_NARROW_PP) {
+            float d = a_lo[k] - b_lo[k];
+            ok = ok && !(d < -r[k] || d > r[k]);
+        } else if (NARROW == TAY_NARROW_PN) {
+            ok = ok && !(a_lo[k] - b_hi[k] > r[k]) && !(b_lo[k] - a_lo[k] > r[k]);
+        } else if (NARROW == TAY_NARROW_NP) {
+            ok = ok && !(a_lo[k] - b_lo[k] > r[k]) && !(b_lo[k] - a_hi[k] > r[k]);
+        } else {
+            ok = ok && !(a_lo[k] - b_hi[k] > r[k]) && !(b_lo[k] - a_hi[k] > r[k]);
+        }
+    }
+    return ok;
+}
+
+template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
+          bool COUNT, class Call>
+__device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
+    static_assert(sizeof(typename T::seer) <= SEER_WORDS * 4u, "seer record larger than the group's agent size");
+    static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
+    constexpr bool SEER_BOX = NARROW == TAY_NARROW_NP || NARROW == TAY_NARROW_NN;
+    constexpr bool SEEN_BOX = NARROW == TAY_NARROW_PN || NARROW == TAY_NARROW_NN;
+
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned tile = tay_xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (tile >= args.tiles)
+        return;
+    const unsigned i0 = tile * 64u;
+    const unsigned i = i0 + lane;
+    const bool active = i < args.seer_count;
+    const unsigned ia = active ? i : i0;
+
+    union {
+        typename T::seer agent;
+        unsigned w[SEER_WORDS];
+    } A;
+    unsigned orig[SEER_WORDS];
+#pragma unroll
+    for (unsigned k = 0; k < SEER_WORDS; ++k)
+        orig[k] = A.w[k] = args.seer_cols[k * args.seer_cap + ia];
+
+    /* position words: 2.. (min corner), 6.. (max corner of a box agent) */
+    float a_lo[3] = {0.0f, 0.0f, 0.0f}, a_hi[3] = {0.0f, 0.0f, 0.0f}, r[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < DIMS; ++k) {
+        a_lo[k] = __uint_as_float(A.w[2 + k]);
+        a_hi[k] = SEER_BOX ? __uint_as_float(A.w[6 + k]) : a_lo[k];
+        r[k] = args.radii[k];
+    }
+    typename T::context *ctx = (typename T::context *)args.context;
+    unsigned pairs = 0;
+
+    auto visit = [&](unsigned beg, unsigned end) {
+        for (unsigned j = beg; j < end; ++j) {
+            union {
+                typename T::seen agent;
+                unsigned w[SEEN_WORDS];
+            } B;
+#pragma unroll
+            for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                B.w[k] = args.seen_cols[k * args.seen_cap + j];
+            float b_lo[3] = {0.0f, 0.0f, 0.0f}, b_hi[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < DIMS; ++k) {
+                b_lo[k] = __uint_as_float(B.w[2 + k]);
+                b_hi[k] = SEEN_BOX ? __uint_as_float(B.w[6 + k]) : b_lo[k];
+            }
+            bool ok = active && (!SKIP_SELF || j != i) && tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
+            if (ok) {
+                call(&A.agent, &B.agent, ctx);
+                if (COUNT)
+                    ++pairs;
+            }
+        }
+    };
+
+    if (SEEN_GRID) {
+        const TayGridGeom *g = args.geom;
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < DIMS; ++k) {
+            float tmin = tay_wave_min(active ? a_lo[k] : __builtin_inff());
+            float tmax = tay_wave_max(active ? a_hi[k] : -__builtin_inff());
+            /* margin (|p| + r) * 2^-20 covers the rounding of p - r and of
+             * the box test, so no cell holding a passing pair is missed */
+            float mlo = (fabsf(tmin) + r[k]) * 0x1p-20f;
+            float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
+            lo[k] = tay_cell_axis((tmin - r[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
+            hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
+        }
+        const unsigned cx = (unsigned)g->count[0];
+        const unsigned cxy = cx * (unsigned)g->count[1];
+        for (int x = lo[0]; x <= hi[0]; ++x)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    const TayCellRange cr = args.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                    visit(cr.beg, cr.end);
+                }
+    } else {
+        visit(0u, args.seen_count);
+    }
+
+    if (active) {
+#pragma unroll
+        for (unsigned k = 0; k < SEER_WORDS; ++k)
+            if (A.w[k] != orig[k])
+                args.seer_out[k * args.seer_cap + i] = A.w[k];
+    }
+    if (COUNT) {
+        unsigned total = tay_wave_sum(pairs);
+        if (lane == 0u && total)
+            atomicAdd(args.pair_count, (unsigned long long)total);
+    }
+}
+
+template <class T, unsigned WORDS, class Call>
+__device__ __forceinline__ void tay_act_agents(const TayActArgs &args, Call call) {
+    static_assert(sizeof(typename T::agent) <= WORDS * 4u, "agent record larger than the group's agent size");
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= args.count)
+        return;
+    union {
+        typename T::agent agent;
+        unsigned w[WORDS];
+    } A;
+    unsigned orig[WORDS];
+#pragma unroll
+    for (unsigned k = 0; k < WORDS; ++k)
+        orig[k] = A.w[k] = args.cols[k * args.cap + i];
+    call(&A.agent, (typename T::context *)args.context);
+#pragma unroll
+    for (unsigned k = 0; k < WORDS; ++k)
+        if (A.w[k] != orig[k])
+            args.cols[k * args.cap + i] = A.w[k];
+}

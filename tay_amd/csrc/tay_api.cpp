@@ -1,0 +1,580 @@
+/*
+ * tay_api.cpp - the Tay public API (include/tay.h) for GPU spaces, built on
+ * the C-ABI of tay_gpu.h.  The state machine follows the reference's
+ * (tay/state.c, tay/state_compile.c): groups own host AoS storage, passes are
+ * registered in order, tay_run compiles on demand, pushes agents and pass
+ * contexts, runs sort -> passes for every step and fetches all agents back.
+ */
+/* Plain C++: this file sees Tay's packed float4, not HIP's vector types, so
+ * it includes no HIP header and reaches the device only through tay_gpu.h. */
+#include "tay.h"
+#include "tay_ext.h"
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#define TAY_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int kMaxGroups = 8;           /* tay/config.h:8 */
+constexpr unsigned kMaxPasses = 32;     /* tay/config.h:9 */
+constexpr unsigned kMaxSources = 16;    /* tay/ocl.h:4 */
+constexpr unsigned kMaxFuncName = 512;  /* tay/state.h:9 */
+constexpr unsigned long long kMB = 1ull << 20;
+
+}  // namespace
+
+struct TayGroup {
+    char *storage;
+    unsigned agent_size;
+    unsigned capacity;
+    int is_point;
+    unsigned id;
+    int active;
+
+    TaySpaceType type;
+    int dims;
+    float4 min_part_sizes;
+    unsigned long long shared_bytes;
+    unsigned count;
+    int box_fixed;
+    float4 box_min, box_max;
+
+    TayGpuSpace *gpu;
+    int push_agents;
+};
+
+struct TayPass {
+    int type;   /* TayGpuPassType */
+    TayGroup *seer;
+    TayGroup *seen;
+    char func_name[kMaxFuncName];
+    float4 radii;
+    int self_see;
+    void *context;
+    unsigned context_size;
+    int has_func;
+};
+
+struct TayState {
+    TayGroup groups[kMaxGroups];
+    TayPass passes[kMaxPasses];
+    unsigned passes_count;
+    int is_running;
+    TayError error;
+    std::string error_message;
+    double ms_per_step;
+    unsigned next_group_id;
+    int recompile;
+    int device_enabled;
+    char *sources[kMaxSources];
+    unsigned sources_count;
+    TayGpuProgram *program;
+    int count_pairs;
+    int timing;
+};
+
+namespace {
+
+void set_error(TayState *state, TayError error, const char *message) {
+    std::fprintf(stderr, "ERROR: %s\n", message);
+    state->error = error;
+    state->error_message = message;
+}
+
+void set_device_error(TayState *state, const char *what) {
+    std::string msg = std::string(what) + ": " + tay_gpu_last_error();
+    set_error(state, TAY_ERROR_OCL, msg.c_str());
+}
+
+/* The GPU space kind a group runs in, or 0 when the group's space type has
+ * no GPU implementation.  After tay_switch_to_ocl the reference runs
+ * TAY_CPU_Z_GRID and TAY_CPU_SIMPLE groups on the device
+ * (tay/state_compile.c:94-107); this library does the same. */
+int gpu_kind(const TayState *state, const TayGroup *g) {
+    switch (g->type) {
+        case TAY_GPU_Z_GRID: return TAY_GPU_KIND_Z_GRID;
+        case TAY_GPU_SIMPLE: return TAY_GPU_KIND_SIMPLE;
+        case TAY_CPU_Z_GRID: return state->device_enabled ? TAY_GPU_KIND_Z_GRID : 0;
+        case TAY_CPU_SIMPLE: return state->device_enabled ? TAY_GPU_KIND_SIMPLE : 0;
+        default: return 0;
+    }
+}
+
+void release_device(TayState *state) {
+    tay_gpu_program_destroy(state->program);
+    state->program = nullptr;
+    for (TayGroup &g : state->groups) {
+        tay_gpu_space_destroy(g.gpu);
+        g.gpu = nullptr;
+    }
+}
+
+/* tay/state_compile.c:5-175 for the device path: validate groups and
+ * passes, create the device spaces and compile the program. */
+int state_compile(TayState *state) {
+    if (!state->recompile)
+        return 1;
+    state->recompile = 0;
+
+    for (TayGroup &g : state->groups) {
+        if (!g.active)
+            continue;
+        if (g.type == TAY_GPU_Z_GRID || g.type == TAY_GPU_SIMPLE)
+            state->device_enabled = tay_gpu_available() || state->device_enabled;
+    }
+    for (TayGroup &g : state->groups) {
+        if (!g.active)
+            continue;
+        const bool grid = g.type == TAY_CPU_GRID || g.type == TAY_CPU_Z_GRID || g.type == TAY_GPU_Z_GRID;
+        if (!g.is_point && grid) {
+            set_error(state, TAY_ERROR_POINT_NONPOINT_MISMATCH, "grid structure can only contain point agents");
+            return 0;
+        }
+        if (g.is_point && g.type == TAY_CPU_AABB_TREE) {
+            set_error(state, TAY_ERROR_POINT_NONPOINT_MISMATCH, "aabb tree structure can only contain non-point agents");
+            return 0;
+        }
+        if (!gpu_kind(state, &g)) {
+            set_error(state, TAY_ERROR_NOT_IMPLEMENTED,
+                      state->device_enabled || g.type == TAY_GPU_Z_GRID || g.type == TAY_GPU_SIMPLE
+                          ? "space type has no GPU implementation (GPU Z-grid or GPU simple)"
+                          : "CPU spaces are not part of this library; use TAY_GPU_Z_GRID or tay_switch_to_ocl()");
+            return 0;
+        }
+    }
+    if (!state->device_enabled && !tay_gpu_available()) {
+        set_error(state, TAY_ERROR_OCL, "device not found");
+        return 0;
+    }
+    state->device_enabled = 1;
+
+    for (unsigned i = 0; i < state->passes_count; ++i) {
+        TayPass &p = state->passes[i];
+        if (p.context && !p.context_size) {
+            set_error(state, TAY_ERROR_OCL, "ocl pass context set but context size is 0");
+            return 0;
+        }
+        if (!p.context && p.context_size) {
+            set_error(state, TAY_ERROR_OCL, "ocl pass context not set but context size is not 0");
+            return 0;
+        }
+        if (p.func_name[0] == '\0') {
+            set_error(state, TAY_ERROR_OCL, "ocl pass with no func name set");
+            return 0;
+        }
+        if (!p.seer || !p.seer->active || (p.type == TAY_GPU_PASS_SEE && (!p.seen || !p.seen->active))) {
+            set_error(state, TAY_ERROR_GROUP_INDEX_OUT_OF_RANGE, "pass refers to a group that does not exist");
+            return 0;
+        }
+        if (p.type == TAY_GPU_PASS_SEE && p.seer->dims != p.seen->dims) {
+            set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "groups with different numbers of dimensions not supported");
+            return 0;
+        }
+    }
+
+    release_device(state);
+    for (TayGroup &g : state->groups) {
+        if (!g.active)
+            continue;
+        float part[4] = {g.min_part_sizes.x, g.min_part_sizes.y, g.min_part_sizes.z, g.min_part_sizes.w};
+        g.gpu = tay_gpu_space_create(gpu_kind(state, &g), g.agent_size, g.capacity, g.is_point, g.dims, part,
+                                     g.shared_bytes);
+        if (!g.gpu) {
+            set_device_error(state, "cannot create the group's device space");
+            release_device(state);
+            return 0;
+        }
+        if (g.box_fixed) {
+            float lo[4] = {g.box_min.x, g.box_min.y, g.box_min.z, g.box_min.w};
+            float hi[4] = {g.box_max.x, g.box_max.y, g.box_max.z, g.box_max.w};
+            tay_gpu_space_fix_box(g.gpu, lo, hi);
+        }
+        g.push_agents = 1;
+    }
+
+    TayGpuPassDesc desc[kMaxPasses];
+    for (unsigned i = 0; i < state->passes_count; ++i) {
+        const TayPass &p = state->passes[i];
+        std::memset(&desc[i], 0, sizeof(desc[i]));
+        desc[i].type = p.type;
+        desc[i].func_name = p.func_name;
+        desc[i].seer = p.seer->gpu;
+        desc[i].seen = p.type == TAY_GPU_PASS_SEE ? p.seen->gpu : nullptr;
+        desc[i].radii[0] = p.radii.x;
+        desc[i].radii[1] = p.radii.y;
+        desc[i].radii[2] = p.radii.z;
+        desc[i].radii[3] = p.radii.w;
+        desc[i].self_see = p.self_see;
+        desc[i].context_size = p.context_size;
+    }
+    static thread_local char log[1 << 16];
+    log[0] = '\0';
+    state->program = tay_gpu_program_create(state->sources, state->sources_count, desc, state->passes_count,
+                                            state->count_pairs, log, sizeof(log));
+    if (!state->program) {
+        if (log[0])
+            std::fprintf(stderr, "%s\n", log);
+        set_device_error(state, "cannot build the program");
+        release_device(state);
+        return 0;
+    }
+    if (state->timing)
+        tay_gpu_program_enable_timing(state->program, 1);
+    return 1;
+}
+
+}  // namespace
+
+TAY_API TayState *tay_create_state() {
+    TayState *state = new TayState();
+    std::memset(state->groups, 0, sizeof(state->groups));
+    std::memset(state->passes, 0, sizeof(state->passes));
+    std::memset(state->sources, 0, sizeof(state->sources));
+    state->passes_count = 0;
+    state->is_running = 0;
+    state->error = TAY_ERROR_NONE;
+    state->ms_per_step = 0.0;
+    state->next_group_id = 0;
+    state->recompile = 1;
+    state->device_enabled = 0;
+    state->sources_count = 0;
+    state->program = nullptr;
+    state->count_pairs = 0;
+    state->timing = 0;
+    return state;
+}
+
+TAY_API void tay_destroy_state(TayState *state) {
+    if (!state)
+        return;
+    release_device(state);
+    for (TayGroup &g : state->groups)
+        std::free(g.storage);
+    for (unsigned i = 0; i < state->sources_count; ++i)
+        std::free(state->sources[i]);
+    delete state;
+}
+
+TAY_API int tay_switch_to_ocl(TayState *state) {
+    if (!state->device_enabled) {
+        state->device_enabled = tay_gpu_available();
+        state->recompile = 1;
+        if (!state->device_enabled)
+            set_error(state, TAY_ERROR_OCL, "device not found");
+    }
+    return state->device_enabled;
+}
+
+TAY_API int tay_switch_to_host(TayState *state) {
+    if (state->device_enabled) {
+        release_device(state);
+        state->device_enabled = 0;
+        state->recompile = 1;
+    }
+    return state->device_enabled;
+}
+
+TAY_API TayError tay_get_error(TayState *state) {
+    return state->error;
+}
+
+TAY_API const char *tay_get_error_message(TayState *state) {
+    return state->error_message.c_str();
+}
+
+/* tay/state.c:73-107 */
+TAY_API TayGroup *tay_add_group(TayState *state, unsigned agent_size, unsigned agent_capacity, int is_point) {
+    int gi = 0;
+    for (; gi < kMaxGroups; ++gi)
+        if (!state->groups[gi].active)
+            break;
+    if (gi == kMaxGroups) {
+        set_error(state, TAY_ERROR_GROUP_INDEX_OUT_OF_RANGE, "no free group slot");
+        return nullptr;
+    }
+    TayGroup *g = state->groups + gi;
+    std::memset(g, 0, sizeof(*g));
+    g->storage = (char *)std::calloc(agent_capacity ? agent_capacity : 1, agent_size ? agent_size : 1);
+    g->agent_size = agent_size;
+    g->capacity = agent_capacity;
+    g->is_point = is_point;
+    g->id = state->next_group_id++;
+    g->active = 1;
+    g->type = TAY_CPU_SIMPLE;
+    g->dims = 3;
+    g->min_part_sizes.x = g->min_part_sizes.y = g->min_part_sizes.z = g->min_part_sizes.w = 1.0f;
+    state->recompile = 1;
+    return g;
+}
+
+/* tay/state.c:109-118 */
+TAY_API void tay_configure_space(TayState *state, TayGroup *group, TaySpaceType space_type, int space_dims,
+                                 float4 min_part_sizes, int shared_size_in_megabytes) {
+    group->type = space_type;
+    group->dims = space_dims;
+    group->min_part_sizes = min_part_sizes;
+    group->shared_bytes = shared_size_in_megabytes > 0 ? (unsigned long long)shared_size_in_megabytes * kMB : 0ull;
+    group->push_agents = 1;
+    state->recompile = 1;
+}
+
+/* tay/state.c:120-125; unlike the reference's OpenCL grid (todo:6) the GPU
+ * Z-grid honours the fixed box. */
+TAY_API void tay_fix_space_box(TayState *state, TayGroup *group, float4 min, float4 max) {
+    (void)state;
+    group->box_min = min;
+    group->box_max = max;
+    group->box_fixed = 1;
+    if (group->gpu) {
+        float lo[4] = {min.x, min.y, min.z, min.w};
+        float hi[4] = {max.x, max.y, max.z, max.w};
+        tay_gpu_space_fix_box(group->gpu, lo, hi);
+    }
+}
+
+static TayPass *new_pass(TayState *state) {
+    if (state->passes_count >= kMaxPasses) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "too many passes");
+        return nullptr;
+    }
+    TayPass *p = state->passes + state->passes_count++;
+    std::memset(p, 0, sizeof(*p));
+    state->recompile = 1;
+    return p;
+}
+
+/* tay/state.c:135-158 */
+TAY_API void tay_add_see(TayState *state, TayGroup *seer_group, TayGroup *seen_group,
+                         void (*func)(void *, void *, void *), char *func_name, float4 radii, int self_see,
+                         void *context, unsigned context_size) {
+    TayPass *p = new_pass(state);
+    if (!p)
+        return;
+    p->type = TAY_GPU_PASS_SEE;
+    p->seer = seer_group;
+    p->seen = seen_group;
+    p->has_func = func != nullptr;
+    std::snprintf(p->func_name, kMaxFuncName, "%s", func_name ? func_name : "");
+    p->radii = radii;
+    p->self_see = self_see;
+    p->context = context;
+    p->context_size = context_size;
+}
+
+TAY_API void tay_add_act(TayState *state, TayGroup *act_group, void (*func)(void *, void *), char *func_name,
+                         void *context, unsigned context_size) {
+    TayPass *p = new_pass(state);
+    if (!p)
+        return;
+    p->type = TAY_GPU_PASS_ACT;
+    p->seer = act_group;
+    p->has_func = func != nullptr;
+    std::snprintf(p->func_name, kMaxFuncName, "%s", func_name ? func_name : "");
+    p->context = context;
+    p->context_size = context_size;
+}
+
+/* tay/state.c:160-181.  Committing agents also marks the group for a push,
+ * so agents added between runs reach the device. */
+TAY_API void tay_clear_all_agents(TayState *state, TayGroup *group) {
+    (void)state;
+    group->count = 0;
+    group->push_agents = 1;
+}
+
+TAY_API void *tay_get_available_agent(TayState *state, TayGroup *group) {
+    if (group->count >= group->capacity) {
+        set_error(state, TAY_ERROR_CAPACITY, "group is full");
+        return nullptr;
+    }
+    return group->storage + (size_t)group->agent_size * group->count;
+}
+
+TAY_API void tay_commit_available_agent(TayState *state, TayGroup *group) {
+    tay_commit_available_agents(state, group, 1);
+}
+
+TAY_API void tay_commit_available_agents(TayState *state, TayGroup *group, unsigned count) {
+    if ((unsigned long long)group->count + count > group->capacity) {
+        set_error(state, TAY_ERROR_CAPACITY, "group is full");
+        return;
+    }
+    group->count += count;
+    group->push_agents = 1;
+}
+
+TAY_API void *tay_get_agent(TayState *state, TayGroup *group, int index) {
+    (void)state;
+    return group->storage + (size_t)group->agent_size * (size_t)index;
+}
+
+TAY_API void tay_simulation_start(TayState *state) {
+    if (state->is_running) {
+        set_error(state, TAY_ERROR_STATE_STATUS, "cannot start simulation if it's already running");
+        return;
+    }
+    state->recompile = 1;
+    state->is_running = 1;
+}
+
+TAY_API void tay_simulation_end(TayState *state) {
+    if (!state->is_running) {
+        set_error(state, TAY_ERROR_STATE_STATUS, "cannot end simulation if it's not running");
+        return;
+    }
+    state->is_running = 0;
+    release_device(state);
+    state->recompile = 1;
+}
+
+/* tay/state.c:194-323 */
+TAY_API int tay_run(TayState *state, int steps) {
+    if (!state_compile(state))
+        return 0;
+    if (!state->is_running) {
+        set_error(state, TAY_ERROR_STATE_STATUS, "cannot run simulation steps if simulation is not started");
+        return 0;
+    }
+    const auto beg = std::chrono::steady_clock::now();
+
+    for (TayGroup &g : state->groups) {
+        if (!g.active || !g.push_agents)
+            continue;
+        if (!tay_gpu_space_push(g.gpu, g.storage, g.count)) {
+            set_device_error(state, "push agents");
+            return 0;
+        }
+        g.push_agents = 0;
+    }
+    for (unsigned i = 0; i < state->passes_count; ++i)
+        if (!tay_gpu_program_set_context(state->program, i, state->passes[i].context)) {
+            set_device_error(state, "push pass context");
+            return 0;
+        }
+    if (state->count_pairs && !tay_gpu_program_reset_pair_counts(state->program)) {
+        set_device_error(state, "reset pair counts");
+        return 0;
+    }
+    if (state->timing && !tay_gpu_program_enable_timing(state->program, 1)) {
+        set_device_error(state, "reset pass timing");
+        return 0;
+    }
+
+    for (int step = 0; step < steps; ++step) {
+        for (TayGroup &g : state->groups)
+            if (g.active && !tay_gpu_space_sort(g.gpu)) {
+                set_device_error(state, "sort");
+                return 0;
+            }
+        for (unsigned i = 0; i < state->passes_count; ++i)
+            if (!tay_gpu_program_run_pass(state->program, i)) {
+                set_device_error(state, "pass");
+                return 0;
+            }
+    }
+
+    for (TayGroup &g : state->groups)
+        if (g.active && !tay_gpu_space_fetch(g.gpu, g.storage, g.count)) {
+            set_device_error(state, "fetch agents");
+            return 0;
+        }
+    if (!tay_gpu_synchronize()) {
+        set_device_error(state, "run");
+        return 0;
+    }
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - beg).count();
+    state->ms_per_step = steps > 0 ? t / (double)steps * 1000.0 : 0.0;
+    return steps;
+}
+
+TAY_API double tay_get_ms_per_step_for_last_run(TayState *state) {
+    return state->ms_per_step;
+}
+
+TAY_API void tay_log(void *file, char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vfprintf(stdout, fmt, ap);
+    va_end(ap);
+    std::fflush(stdout);
+    if (file) {
+        va_start(ap, fmt);
+        std::vfprintf((FILE *)file, fmt, ap);
+        va_end(ap);
+    }
+}
+
+/* tay/ocl.c:274-282 */
+TAY_API void ocl_add_source(TayState *state, const char *source) {
+    if (state->sources_count < kMaxSources && source) {
+        state->sources[state->sources_count++] = strdup(source);
+        state->recompile = 1;
+    }
+}
+
+/* The device path needs no host worker threads; the pool calls are kept so
+ * that hosts written for the reference link unchanged. */
+static unsigned g_threads = 0;
+
+TAY_API void tay_threads_start(unsigned threads_count, unsigned thread_storage_size) {
+    (void)thread_storage_size;
+    g_threads = threads_count ? (threads_count > 64 ? 64 : threads_count) : 1;
+}
+
+TAY_API void tay_threads_stop() {
+    g_threads = 0;
+}
+
+TAY_API unsigned tay_get_number_of_threads() {
+    return g_threads;
+}
+
+/* ------------------------------------------------------------ extensions */
+
+TAY_API void tay_set_pair_counting(TayState *state, int enable) {
+    if (state->count_pairs != (enable != 0)) {
+        state->count_pairs = enable != 0;
+        state->recompile = 1;
+    }
+}
+
+TAY_API int tay_get_pair_count(TayState *state, unsigned pass_index, unsigned long long *pairs) {
+    if (!state->program) {
+        set_error(state, TAY_ERROR_STATE_STATUS, "no program: run the simulation first");
+        return 0;
+    }
+    if (!tay_gpu_program_pair_count(state->program, pass_index, pairs)) {
+        set_device_error(state, "pair count");
+        return 0;
+    }
+    return 1;
+}
+
+TAY_API void tay_set_pass_timing(TayState *state, int enable) {
+    state->timing = enable != 0;
+    if (state->program)
+        tay_gpu_program_enable_timing(state->program, state->timing);
+}
+
+TAY_API int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, unsigned *launches) {
+    if (!state->program) {
+        set_error(state, TAY_ERROR_STATE_STATUS, "no program: run the simulation first");
+        return 0;
+    }
+    if (!tay_gpu_program_pass_time(state->program, pass_index, ms, launches)) {
+        set_device_error(state, "pass time");
+        return 0;
+    }
+    return 1;
+}
+
+TAY_API TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group) {
+    (void)state;
+    return group->gpu;
+}

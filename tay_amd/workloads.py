@@ -1,0 +1,259 @@
+"""Synthetic workloads and a state builder over the Tay API.
+
+Inputs come from numpy's PCG64 generator with a fixed seed, so the GPU
+library and the CPU oracle receive identical agents.  A Simulation builds one
+state on any library exporting the Tay API, the way the reference harness
+builds its test states (test/test_basic.c:12-85): add sources, one group,
+configure the space, register passes, add agents, start.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import agents as A
+from .api import float4
+
+
+def rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def unit_vectors(n, gen):
+    v = gen.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v.astype(np.float32)
+
+
+def uniform_points(n, lo, hi, gen):
+    return gen.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+
+
+def clustered_points(n, clusters, sigma, box, gen):
+    """n points from `clusters` isotropic Gaussians of deviation sigma whose
+    centres are uniform in [3 sigma, box - 3 sigma)^3."""
+    centres = gen.uniform(3 * sigma, box - 3 * sigma, size=(clusters, 3))
+    which = gen.integers(0, clusters, size=n)
+    return (centres[which] + gen.normal(scale=sigma, size=(n, 3))).astype(np.float32)
+
+
+def _xyz0(a):
+    out = np.zeros((len(a), 4), np.float32)
+    out[:, :3] = a
+    return out
+
+
+def plain_agents(n, seed=1, box=1000.0):
+    """Plain workload: positions U[0, box)^3, velocities U[-1, 1]^3."""
+    g = rng(seed)
+    rec = np.zeros(n, A.PLAIN_AGENT)
+    rec["p"] = _xyz0(uniform_points(n, 0.0, box, g))
+    rec["v"] = _xyz0(g.uniform(-1.0, 1.0, size=(n, 3)).astype(np.float32))
+    rec["result_index"] = np.arange(n)
+    return rec
+
+
+def test_agents(n, seed=1, box=1000.0, clump=0.0):
+    """The reference harness's agents (test/test.c:317-348): positions
+    uniform in [0, box)^3, unit-speed random velocities; with clump > 0 that
+    fraction sits in [0, 0.05 box)^3 (test/test_basic.c:57-69)."""
+    g = rng(seed)
+    rec = np.zeros(n, A.AGENT)
+    k = int(n * clump)
+    pos = np.concatenate([uniform_points(n - k, 0.0, box, g), uniform_points(k, 0.0, 0.05 * box, g)])
+    rec["p"] = _xyz0(pos)
+    rec["v"] = _xyz0(unit_vectors(n, g))
+    rec["result_index"] = np.arange(n)
+    return rec
+
+
+def flock_agents(n, seed=1, clusters=64, sigma=30.0, box=2000.0):
+    """Flocking workload: Gaussian clusters, unit-random velocities."""
+    g = rng(seed)
+    rec = np.zeros(n, A.FLOCK_AGENT)
+    rec["p"] = _xyz0(clustered_points(n, clusters, sigma, box, g))
+    rec["v"] = _xyz0(unit_vectors(n, g))
+    rec["result_index"] = np.arange(n)
+    return rec
+
+
+def boids(n, seed=1, half=300.0):
+    """m_flocking's boids (m_flocking/main.c:161-194): positions U[-half,
+    half)^3, unit headings, speed 1."""
+    g = rng(seed)
+    rec = np.zeros(n, A.BOID)
+    rec["p"] = _xyz0(uniform_points(n, -half, half, g))
+    rec["dir"] = _xyz0(unit_vectors(n, g))
+    rec["speed"] = 1.0
+    return rec
+
+
+def sph_particles(n, h=1.0, fill=0.4, seed=1, jitter=0.05):
+    """Dam break: a water block filling `fill` of a box on a lattice of
+    spacing 0.5 h, jittered by +-jitter h.  Returns (records, box_min,
+    box_max)."""
+    g = rng(seed)
+    spacing = 0.5 * h
+    per_axis = int(np.ceil(n ** (1.0 / 3.0)))
+    side = per_axis * spacing
+    box = side / fill ** (1.0 / 3.0)
+    idx = np.arange(per_axis ** 3)[:n]
+    lattice = np.stack([idx % per_axis, (idx // per_axis) % per_axis, idx // (per_axis * per_axis)], 1)
+    pos = (lattice + 0.5) * spacing + g.uniform(-jitter * h, jitter * h, size=(n, 3))
+    rec = np.zeros(n, A.SPH_PARTICLE)
+    rec["p"] = _xyz0(pos.astype(np.float32))
+    return rec, np.array([0, 0, 0, 0], np.float32), np.array([box, box, box, 0], np.float32)
+
+
+def sph_context(h=1.0, dt=1e-3, box_min=None, box_max=None, m=0.05, rest_density=998.29):
+    """Kernel constants of Mueller, Charypar and Gross (2003), computed as in
+    m_sph/main.c:172-187."""
+    c = np.zeros(1, A.SPH_CONTEXT)
+    pi = np.float32(3.14159265358979323846)
+    h = np.float32(h)
+    h2 = h * h
+    h6 = h2 * h2 * h2
+    h9 = h6 * h2 * h
+    m = np.float32(m)
+    c["h"] = h
+    c["h2"] = h2
+    c["dt"] = dt
+    c["m"] = m
+    c["dynamic_viscosity"] = 3.5
+    c["K"] = np.float32(101325.0) / np.float32(rest_density)
+    c["density"] = rest_density
+    c["poly6"] = m * np.float32(315.0) / (np.float32(64.0) * pi * h9)
+    c["poly6_gradient"] = m * np.float32(-945.0) / (np.float32(32.0) * pi * h9)
+    c["poly6_laplacian"] = c["poly6_gradient"]
+    c["spiky"] = m * np.float32(-45.0) / (pi * h6)
+    c["viscosity"] = -c["spiky"]
+    c["min"] = box_min if box_min is not None else 0.0
+    c["max"] = box_max if box_max is not None else 0.0
+    return c
+
+
+@dataclass
+class Pass:
+    kind: str                   # "see" or "act"
+    func: str
+    context: np.ndarray = None  # one-element structured array, or None
+    radius: float = 0.0
+    self_see: int = 0
+
+
+@dataclass
+class Model:
+    dtype: np.dtype
+    passes: list
+    part: float                 # suggested cell size
+    dims: int = 3
+    extra: dict = field(default_factory=dict)
+
+
+def plain_model(radius=40.0, box=1000.0):
+    see = np.zeros(1, A.PLAIN_SEE_CONTEXT)
+    see["r2"] = np.float32(radius) * np.float32(radius)
+    act = np.zeros(1, A.ACT_CONTEXT)
+    act["max"] = [box, box, box, 0.0]
+    return Model(A.PLAIN_AGENT, [Pass("see", "plain_see", see, radius), Pass("act", "plain_act", act)],
+                 part=2.0 * radius)
+
+
+def test_model(radius=50.0, box=1000.0, part=None):
+    see = np.zeros(1, A.SEE_CONTEXT)
+    see["radii"] = [radius, radius, radius, 0.0]
+    act = np.zeros(1, A.ACT_CONTEXT)
+    act["max"] = [box, box, box, 0.0]
+    return Model(A.AGENT, [Pass("see", "agent_see", see, radius), Pass("act", "agent_act", act)],
+                 part=2.0 * radius if part is None else part)
+
+
+def flock_model(radius=25.0, weights=(1.5, 1.0, 1.0), dt=0.1, max_speed=2.0, part=None):
+    see = np.zeros(1, A.FLOCK_SEE_CONTEXT)
+    see["r"] = radius
+    see["separation_r"] = np.float32(radius) * np.float32(0.5)
+    act = np.zeros(1, A.FLOCK_ACT_CONTEXT)
+    act["separation_w"], act["alignment_w"], act["cohesion_w"] = weights
+    act["dt"] = dt
+    act["max_speed"] = max_speed
+    return Model(A.FLOCK_AGENT, [Pass("see", "flock_see", see, radius), Pass("act", "flock_act", act)],
+                 part=radius if part is None else part)
+
+
+def boid_model(radius=20.0):
+    see = np.zeros(1, A.BOID_SEE_CONTEXT)
+    see["r"] = radius
+    see["separation_r"] = np.float32(radius) * np.float32(0.5)
+    act = np.zeros(1, A.BOID_ACT_CONTEXT)
+    return Model(A.BOID, [Pass("see", "boid_see", see, radius), Pass("act", "boid_act", act)], part=radius)
+
+
+def sph_model(ctx, h=1.0):
+    return Model(A.SPH_PARTICLE,
+                 [Pass("see", "sph_density", ctx, h, self_see=1), Pass("act", "sph_pressure", ctx),
+                  Pass("see", "sph_forces", ctx, h, self_see=0), Pass("act", "sph_integrate", ctx)],
+                 part=h)
+
+
+class Simulation:
+    """One Tay state with one group running `model` on `lib`.
+
+    funcs is a loaded library holding the model's C callbacks (the CPU
+    oracle needs them; the GPU library compiles the sources instead)."""
+
+    def __init__(self, lib, model, records, space_type, funcs=None, shared_mb=250, capacity=None,
+                 sources=None, part=None, box=None):
+        self.lib = lib
+        self.model = model
+        self.state = lib.tay_create_state()
+        if not self.state:
+            raise RuntimeError("tay_create_state failed")
+        for text in (A.source_texts() if sources is None else sources):
+            lib.ocl_add_source(self.state, text.encode())
+        n = len(records)
+        self.group = lib.tay_add_group(self.state, model.dtype.itemsize, capacity or max(n, 1), 1)
+        p = model.part if part is None else part
+        lib.tay_configure_space(self.state, self.group, space_type, model.dims, float4(p, p, p, p), shared_mb)
+        if box is not None:
+            lib.tay_fix_space_box(self.state, self.group, float4(*box[0]), float4(*box[1]))
+        self._keep = []
+        for ps in model.passes:
+            ctx = ps.context
+            ptr = ctx.ctypes.data if ctx is not None else None
+            size = ctx.itemsize if ctx is not None else 0
+            self._keep.append(ctx)
+            fn = C.cast(getattr(funcs, ps.func), C.c_void_p) if funcs is not None else None
+            if ps.kind == "see":
+                r = ps.radius
+                lib.tay_add_see(self.state, self.group, self.group, fn, ps.func.encode(), float4(r, r, r, r),
+                                ps.self_see, ptr, size)
+            else:
+                lib.tay_add_act(self.state, self.group, fn, ps.func.encode(), ptr, size)
+        lib.add_agents(self.state, self.group, records)
+        self.count = n
+        lib.tay_simulation_start(self.state)
+
+    def run(self, steps):
+        done = self.lib.tay_run(self.state, steps)
+        if done != steps:
+            raise RuntimeError("tay_run failed: error %d %s" % (self.lib.tay_get_error(self.state),
+                                                                 self.lib.error_message(self.state)))
+        return done
+
+    def agents(self):
+        return self.lib.get_agents(self.state, self.group, self.count, self.model.dtype)
+
+    def ms_per_step(self):
+        return self.lib.tay_get_ms_per_step_for_last_run(self.state)
+
+    def close(self):
+        if self.state:
+            self.lib.tay_simulation_end(self.state)
+            self.lib.tay_destroy_state(self.state)
+            self.state = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

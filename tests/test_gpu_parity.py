@@ -1,0 +1,170 @@
+"""GPU Z-grid parity against the CPU oracle on identical seeded inputs.
+
+The oracle's TAY_GPU_Z_GRID space is the CPU statement of what the GPU grid
+computes (geometry of the reference's OpenCL grid, Morton keys, stable key
+order, x->y->z cell sweep), so with identical inputs every integer output
+(cell keys, ranks, storage order, neighbour counts) and every float output
+(sums accumulated in the same pair order, positions after the act pass) must
+match bit for bit.  Neighbour counts are also checked against CpuSimple, the
+all-pairs definition of the pair set.
+"""
+import numpy as np
+import pytest
+
+from tay_amd import agents as A
+from tay_amd import api
+from tay_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(gpu, oracle, models, model, records, steps, space=api.TAY_GPU_Z_GRID, oracle_space=None, **kw):
+    """Run the same scenario on the GPU library and the oracle; return both
+    agent arrays (storage order) and the simulations' pair counts."""
+    print("case: %s n=%d steps=%d" % (model.passes[0].func, len(records), steps), flush=True)
+    with W.Simulation(gpu, model, records, space, **kw) as g:
+        gpu.tay_set_pair_counting(g.state, 1)
+        g.run(steps)
+        ga = g.agents()
+        gpairs = [gpu.pair_counts(g.state, i)[1] for i, p in enumerate(model.passes) if p.kind == "see"]
+    with W.Simulation(oracle, model, records, oracle_space or api.TAY_GPU_Z_GRID, funcs=models, **kw) as o:
+        o.run(steps)
+        oa = o.agents()
+        opairs = [oracle.pair_counts(o.state, i)[1] for i, p in enumerate(model.passes) if p.kind == "see"]
+    return ga, oa, gpairs, opairs
+
+
+def assert_bitwise(a, b, fields=None):
+    assert a.dtype == b.dtype and len(a) == len(b)
+    for f in fields or a.dtype.names:
+        x, y = a[f], b[f]
+        if x.dtype.kind == "f":
+            x, y = x.view(np.uint32), y.view(np.uint32)
+        bad = np.nonzero((x != y).reshape(len(a), -1).any(axis=1))[0]
+        assert len(bad) == 0, "field %s differs at %d agents, first %s: gpu %s oracle %s" % (
+            f, len(bad), bad[:5], a[f][bad[:3]], b[f][bad[:3]])
+
+
+def xyz_fields(dtype):
+    """Fields to compare bitwise; the unused .w lane of float4 fields is
+    compared too since every model helper writes it as 0."""
+    return list(dtype.names)
+
+
+def test_plain_one_step_bit_exact(gpu, oracle, models):
+    recs = W.plain_agents(10000, seed=11)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(40.0), recs, 1)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_plain_trajectory_20_steps_bit_exact(gpu, oracle, models):
+    recs = W.plain_agents(10000, seed=12)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(40.0), recs, 20)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_plain_counts_match_all_pairs(gpu, oracle, models):
+    recs = W.plain_agents(4000, seed=13)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(60.0), recs, 1,
+                              oracle_space=api.TAY_CPU_SIMPLE)
+    ga = ga[np.argsort(ga["result_index"])]
+    oa = oa[np.argsort(oa["result_index"])]
+    assert np.array_equal(ga["last_ncount"], oa["last_ncount"])
+    assert gp == op
+    # a symmetric test counts every unordered pair twice
+    assert ga["last_ncount"].sum() % 2 == 0
+
+
+def test_reference_test_model_bit_exact(gpu, oracle, models):
+    recs = W.test_agents(10000, seed=1)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.test_model(50.0), recs, 5)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_flocking_clustered_bit_exact(gpu, oracle, models):
+    recs = W.flock_agents(30000, seed=5, clusters=8, sigma=30.0, box=600.0)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.flock_model(25.0), recs, 3)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_grid_geometry_bit_exact(gpu, oracle, models):
+    recs = W.flock_agents(5000, seed=6, clusters=4, sigma=40.0, box=800.0)
+    model = W.flock_model(25.0)
+    out = []
+    for lib, funcs in ((gpu, None), (oracle, models)):
+        with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, funcs=funcs) as s:
+            s.run(1)
+            out.append(lib.grid_info(s.state, s.group))
+    (o1, s1, c1, n1), (o2, s2, c2, n2) = out
+    assert np.array_equal(o1[:3].view(np.uint32), o2[:3].view(np.uint32))
+    assert np.array_equal(s1[:3].view(np.uint32), s2[:3].view(np.uint32))
+    assert np.array_equal(c1[:3], c2[:3]) and n1 == n2
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4097])
+def test_ragged_sizes(gpu, oracle, models, n):
+    recs = W.plain_agents(n, seed=100 + n, box=200.0)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(30.0), recs, 2)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_all_agents_in_one_cell(gpu, oracle, models):
+    g = W.rng(7)
+    recs = W.plain_agents(3000, seed=7)
+    recs["p"][:, :3] = g.uniform(0.0, 5.0, size=(3000, 3)).astype(np.float32)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(40.0), recs, 2)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_agents_on_cell_boundaries(gpu, oracle, models):
+    # positions on a lattice of the cell size: many land exactly on faces
+    g = W.rng(8)
+    recs = W.plain_agents(5000, seed=8)
+    recs["p"][:, :3] = (g.integers(0, 13, size=(5000, 3)) * 80.0).astype(np.float32)
+    recs["v"][:] = 0.0
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(40.0), recs, 2)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_sph_two_see_passes_bit_exact(gpu, oracle, models):
+    recs, lo, hi = W.sph_particles(8000, h=1.0)
+    ctx = W.sph_context(h=1.0, dt=1e-3, box_min=lo, box_max=hi)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.sph_model(ctx, 1.0), recs, 3)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_gpu_simple_space_matches_oracle_simple(gpu, oracle, models):
+    recs = W.plain_agents(3000, seed=9)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(50.0), recs, 2,
+                              space=api.TAY_GPU_SIMPLE, oracle_space=api.TAY_CPU_SIMPLE)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
+def test_empty_group_runs(gpu):
+    with W.Simulation(gpu, W.plain_model(40.0), W.plain_agents(0), api.TAY_GPU_Z_GRID, capacity=16) as s:
+        s.run(3)
+        assert len(s.agents()) == 0
+
+
+def test_sorted_storage_properties_large(gpu):
+    """1M clustered agents: the storage is a permutation of the input,
+    ordered by Morton key, ranks count up inside each cell."""
+    recs = W.flock_agents(1_000_000, seed=21)
+    with W.Simulation(gpu, W.flock_model(25.0), recs, api.TAY_GPU_Z_GRID) as s:
+        s.run(1)
+        a = s.agents()
+    assert np.array_equal(np.sort(a["result_index"]), np.arange(len(recs)))
+    keys = a["part_i"].astype(np.int64)
+    assert np.all(np.diff(keys) >= 0)
+    starts = np.r_[0, np.nonzero(np.diff(keys))[0] + 1]
+    expect = np.arange(len(a)) - np.repeat(starts, np.diff(np.r_[starts, len(a)]))
+    assert np.array_equal(a["cell_agent_i"], expect)
