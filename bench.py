@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPU Z-grid step on one MI355X.
+
+Workload: 3D flocking, 1,000,000 agents from 64 Gaussian clusters (sigma 30)
+in a 2000^3 box, unit-random velocities, see radius 25, separation /
+alignment / cohesion weights 1.5 / 1.0 / 1.0, dt 0.1, speed clamp 2.0, cell
+size 25.  A step is the grid rebuild, the see pass and the act pass.
+
+value = see-pass pair interactions/s: P_see (agent_see calls, i.e. pairs
+passing the box test, counted on the device) divided by the see kernels'
+HIP-event time.  steps/s = timed steps / wall time of tay_run with agents
+resident on the device (no fetch inside the timed region).  P_see comes from
+a counting twin of the timed state: the GPU path is deterministic, so the
+twin walks the identical trajectory.
+
+CPU baseline: the oracle's restatement of the reference's threaded CpuGrid
+(tay/cpu_grid.c) on the same input, a bounded sample of steps.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "see-pass pair interactions/s + steps/s, batched vs plain Z-grid, 1 MI355X"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+# FP32 operations every box-passing pair executes in flock_see: box test
+# (3 subtractions, 6 compares), offset (3), length (3 mul, 2 add, 1 sqrt) and
+# the cutoff compare (1).
+F_PAIR = 19
+# Compulsory bytes of one see launch per agent: the seer reads p, v and the
+# three accumulators (x, y, z) plus both counts (17 words) and writes the
+# accumulators and counts (11 words); the seen side reads p and v (6 words).
+B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(recs, model, threads, steps):
+    from tay_amd import api
+    from tay_amd import workloads as W
+    oracle = api.TayLib(os.path.join(ROOT, "oracle", "_build", "liboracle.so"))
+    models = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "libmodels.so"))
+    oracle.tay_threads_start(threads, 0)
+    with W.Simulation(oracle, model, recs, api.TAY_CPU_GRID, funcs=models) as s:
+        t0 = time.perf_counter()
+        s.run(steps)
+        wall = time.perf_counter() - t0
+        see_s = oracle.oracle_get_see_seconds(s.state)
+        _, narrow = oracle.pair_counts(s.state, 0)
+    oracle.tay_threads_stop()
+    return {
+        "value": narrow / see_s,
+        "unit": "pair interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d step(s) of CpuGrid (AoS, serial sort, threaded see/act, indirect call per pair) on the "
+                  "same %d agents; %.1f s wall, %.3f steps/s; host %s, nproc %d"
+                  % (steps, len(recs), wall, steps / wall, cpu_model(), os.cpu_count() or 0),
+        "steps_per_s": steps / wall,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--agents", type=int, default=1_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the Z-grid runs on a single MI355X in this build; "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    from tay_amd import api
+    from tay_amd import workloads as W
+
+    lib = api.TayLib(api.LIBRARY)
+    if not lib.tay_gpu_available():
+        raise SystemExit("no GPU: " + lib.tay_gpu_last_error().decode())
+
+    recs = W.flock_agents(args.agents, seed=1, clusters=64, sigma=30.0, box=2000.0)
+    model = W.flock_model(radius=25.0, weights=(1.5, 1.0, 1.0), dt=0.1, max_speed=2.0)
+
+    # counting twin: same inputs, same trajectory, pair counts of the timed steps
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as twin:
+        lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
+        lib.tay_set_pair_counting(twin.state, 1)
+        twin.run(args.warmup) if args.warmup else None
+        twin.run(args.steps)
+        p_see = lib.pair_counts(twin.state, 0)[1]
+
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
+        lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
+        if args.warmup:
+            sim.run(args.warmup)
+        lib.tay_set_pass_timing(sim.state, 1)
+        lib.tay_gpu_synchronize()
+        t0 = time.perf_counter()
+        sim.run(args.steps)
+        lib.tay_gpu_synchronize()
+        wall = time.perf_counter() - t0
+        see_ms, see_launches = lib.pass_time(sim.state, 0)
+        act_ms, act_launches = lib.pass_time(sim.state, 1)
+
+    steps_per_s = args.steps / wall
+    see_s = see_ms / 1e3
+    pairs_per_s = p_see / see_s
+    see_avg_s = see_s / see_launches
+    flop_per_launch = p_see / args.steps * F_PAIR
+    achieved = flop_per_launch / see_avg_s / 1e12
+    hbm_achieved = B_SEE_PER_AGENT * args.agents / see_avg_s / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": pairs_per_s,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "flocking-1M", "agents": args.agents, "clusters": 64, "sigma": 30.0,
+                   "box": 2000.0, "radius": 25.0, "cell_size": 25.0, "weights": [1.5, 1.0, 1.0], "dt": 0.1,
+                   "max_speed": 2.0, "structure": "TAY_GPU_Z_GRID"},
+        "steps_per_s": steps_per_s,
+        "p_see_per_step": p_see / args.steps,
+        "see_ms_per_step": see_ms / see_launches,
+        "act_ms_per_step": act_ms / max(act_launches, 1),
+        "build_ms_per_step": max(wall * 1e3 / args.steps - see_ms / see_launches - act_ms / max(act_launches, 1),
+                                 0.0),
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": "see (tay_pass_0, flock_see)",
+            "note": "FP32 vector-ALU bound, no matrix instructions: the peak is MI355X's FP32 157.3 TFLOP/s "
+                    "(FMA counted as 2, packed dual-issue); F = P_see x %d FP32 ops" % F_PAIR,
+        },
+        "roofline_hbm": {"achieved": hbm_achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": hbm_achieved / PEAK_HBM_GBS, "bytes_per_agent": B_SEE_PER_AGENT},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(recs, model, args.cpu_threads, args.cpu_steps)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -36,6 +36,22 @@ int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, unsigned
  * the first tay_run and for groups not on the GPU). */
 TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group);
 
+/* When agents come back to host storage.  TAY_FETCH_EVERY_RUN (default) is
+ * the reference's behaviour: every tay_run ends by copying all agents of all
+ * groups to the host (tay/state.c:313-315).  TAY_FETCH_ON_DEMAND leaves them
+ * on the device until tay_fetch_agents() is called, so short runs in an
+ * interactive loop do not pay a full copy each time. */
+typedef enum TayFetchMode {
+    TAY_FETCH_EVERY_RUN = 0,
+    TAY_FETCH_ON_DEMAND = 1,
+} TayFetchMode;
+
+void tay_set_fetch_mode(TayState *state, int mode);
+
+/* Copies the group's agents to host storage now (all groups when group is
+ * NULL); returns 1 on success. */
+int tay_fetch_agents(TayState *state, TayGroup *group);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,10 @@ TAY_ERROR_NOT_IMPLEMENTED = 5
 TAY_ERROR_OCL = 6
 TAY_ERROR_CAPACITY = 7
 
+# TayFetchMode (include/tay_ext.h)
+TAY_FETCH_EVERY_RUN = 0
+TAY_FETCH_ON_DEMAND = 1
+
 # TayGpuSpaceKind / TayGpuPassType (include/tay_gpu.h)
 TAY_GPU_KIND_Z_GRID = 1
 TAY_GPU_KIND_SIMPLE = 2
@@ -94,6 +98,8 @@ _GPU = {
     "tay_set_pass_timing": (None, [_vp, _i]),
     "tay_get_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_get_gpu_space": (_vp, [_vp, _vp]),
+    "tay_set_fetch_mode": (None, [_vp, _i]),
+    "tay_fetch_agents": (_i, [_vp, _vp]),
     "tay_gpu_available": (_i, []),
     "tay_gpu_last_error": (C.c_char_p, []),
     "tay_gpu_synchronize": (_i, []),

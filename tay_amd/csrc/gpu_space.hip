@@ -107,7 +107,7 @@ constexpr unsigned kRadixPasses = 4;            /* 32-bit keys */
 constexpr unsigned kRadixWaves = kBuildBlock / 64;
 constexpr unsigned kRadixRounds = 16;
 constexpr unsigned kRadixTile = kRadixWaves * kRadixRounds * 64;   /* keys per block */
-constexpr unsigned kScanThreads = 1024;
+constexpr unsigned kScanBlocks = 64;             /* segments of the histogram scan, at most one wavefront's lanes */
 
 __device__ __forceinline__ unsigned compact2(unsigned v) {
     v &= 0x55555555u;
@@ -343,28 +343,73 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_hist(const unsigned *__re
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-/* Exclusive scan of total entries in place, one block. */
-__global__ __launch_bounds__(kScanThreads) void k_radix_scan(unsigned *__restrict__ hist, unsigned total) {
-    __shared__ unsigned s[kScanThreads];
-    const unsigned t = threadIdx.x;
-    const unsigned per = (total + kScanThreads - 1) / kScanThreads;
-    const unsigned beg = min(t * per, total), end = min(beg + per, total);
-    unsigned sum = 0;
-    for (unsigned i = beg; i < end; ++i)
-        sum += hist[i];
-    s[t] = sum;
-    __syncthreads();
-    for (unsigned off = 1; off < kScanThreads; off <<= 1) {
-        unsigned v = t >= off ? s[t - off] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
+/* Exclusive scan of the digit-major histogram in place, in three steps over
+ * kScanBlocks contiguous segments: segment sums, a one-wavefront scan of the
+ * sums, then each segment scanned in coalesced rounds from its carry-in. */
+__device__ __forceinline__ unsigned wave_inclusive_sum(unsigned v, unsigned lane) {
+    for (unsigned o = 1; o < 64; o <<= 1) {
+        unsigned u = __shfl_up(v, o);
+        v += lane >= o ? u : 0u;
     }
-    unsigned run = s[t] - sum;
-    for (unsigned i = beg; i < end; ++i) {
-        unsigned v = hist[i];
-        hist[i] = run;
-        run += v;
+    return v;
+}
+
+__device__ __forceinline__ unsigned segment_begin(unsigned s, unsigned total, unsigned nseg) {
+    return (unsigned)(((unsigned long long)total * s) / nseg);
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_scan_reduce(const unsigned *__restrict__ hist, unsigned total,
+                                                             unsigned *__restrict__ sums) {
+    __shared__ unsigned s[kBuildBlock / 64];
+    const unsigned beg = segment_begin(blockIdx.x, total, gridDim.x);
+    const unsigned end = segment_begin(blockIdx.x + 1, total, gridDim.x);
+    unsigned v = 0;
+    for (unsigned i = beg + threadIdx.x; i < end; i += kBuildBlock)
+        v += hist[i];
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63u) == 0)
+        s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (unsigned w = 0; w < kBuildBlock / 64; ++w)
+            t += s[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_scan_top(unsigned *__restrict__ sums, unsigned n) {
+    const unsigned lane = threadIdx.x;
+    const unsigned v = lane < n ? sums[lane] : 0u;
+    const unsigned inc = wave_inclusive_sum(v, lane);
+    if (lane < n)
+        sums[lane] = inc - v;
+}
+
+__global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict__ hist, unsigned total,
+                                                           const unsigned *__restrict__ sums) {
+    __shared__ unsigned s[kBuildBlock / 64];
+    const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const unsigned beg = segment_begin(blockIdx.x, total, gridDim.x);
+    const unsigned end = segment_begin(blockIdx.x + 1, total, gridDim.x);
+    unsigned carry = sums[blockIdx.x];
+    for (unsigned base = beg; base < end; base += kBuildBlock) {
+        const unsigned i = base + threadIdx.x;
+        const unsigned v = i < end ? hist[i] : 0u;
+        const unsigned inc = wave_inclusive_sum(v, lane);
+        if (lane == 63u)
+            s[w] = inc;
+        __syncthreads();
+        unsigned before = 0, round = 0;
+        for (unsigned k = 0; k < kBuildBlock / 64; ++k) {
+            before += k < w ? s[k] : 0u;
+            round += s[k];
+        }
+        if (i < end)
+            hist[i] = carry + before + inc - v;
+        carry += round;
+        __syncthreads();
     }
 }
 
@@ -538,6 +583,7 @@ TAY_API void tay_gpu_space_destroy(TayGpuSpace *sp) {
     device_free(sp->vals[0]);
     device_free(sp->vals[1]);
     device_free(sp->block_hist);
+    device_free(sp->scan_sums);
     device_free(sp->ranges);
     device_free(sp->staging);
     delete sp;
@@ -604,6 +650,7 @@ TAY_API TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigne
              device_alloc((void **)&sp->vals[1], 4ull * sp->capacity, "sort values") &&
              device_alloc((void **)&sp->block_hist, 4ull * kRadix * (sp->radix_blocks ? sp->radix_blocks : 1),
                           "radix histogram") &&
+             device_alloc((void **)&sp->scan_sums, 4ull * kScanBlocks, "scan sums") &&
              device_alloc((void **)&sp->ranges, sizeof(TayCellRange) * sp->max_cells, "cell table");
         if (ok)
             ok = hip_ok(hipMemsetAsync(sp->geom, 0, sizeof(TayGridGeom), stream()), "hipMemsetAsync");
@@ -706,8 +753,12 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
     for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
         const unsigned shift = pass * kRadixBits;
         const unsigned in = pass & 1u, out = in ^ 1u;
+        const unsigned total = kRadix * nblocks;
+        const unsigned segs = total < kScanBlocks * kBuildBlock ? (total + kBuildBlock - 1) / kBuildBlock : kScanBlocks;
         k_radix_hist<<<nblocks, kBuildBlock, 0, s>>>(sp->keys[in], n, shift, sp->block_hist, nblocks);
-        k_radix_scan<<<1, kScanThreads, 0, s>>>(sp->block_hist, kRadix * nblocks);
+        k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
+        k_scan_top<<<1, 64, 0, s>>>(sp->scan_sums, segs);
+        k_scan_down<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
         k_radix_scatter<<<nblocks, kBuildBlock, 0, s>>>(sp->keys[in], sp->vals[in], sp->keys[out], sp->vals[out], n,
                                                         shift, sp->block_hist, nblocks, pass == 0);
     }

@@ -59,6 +59,7 @@ struct TayGpuSpace {
     unsigned *keys[2];
     unsigned *vals[2];
     unsigned *block_hist;
+    unsigned *scan_sums;
     unsigned radix_blocks;
     TayCellRange *ranges;
     int box_fixed;

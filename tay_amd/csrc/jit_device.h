@@ -34,6 +34,8 @@ template <typename A, typename C> struct tay_act_traits<void(A *, C *)> {
 
 enum { TAY_NARROW_PP = 0, TAY_NARROW_PN = 1, TAY_NARROW_NP = 2, TAY_NARROW_NN = 3 };
 
+#define TAY_SEE_BLOCK 256u
+
 __device__ __forceinline__ float tay_wave_min(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
@@ -72,6 +74,10 @@ __device__ __forceinline__ int tay_cell_axis(float p, float origin, float size, 
     return c > count - 1 ? count - 1 : c;
 }
 
+/* The reference's per-axis tests (tay/space.c:62-104), evaluated without
+ * short-circuit branches.  For point pairs the reference skips when
+ * d < -r || d > r; !(|d| > r) is the same predicate for every d and r,
+ * NaN and negative radii included, and costs one compare. */
 template <int NARROW, int DIMS>
 __device__ __forceinline__ bool tay_narrow(const float *a_lo, const float *a_hi, const float *b_lo, const float *b_hi,
                                            const float *r) {
@@ -79,19 +85,34 @@ __device__ __forceinline__ bool tay_narrow(const float *a_lo, const float *a_hi,
 #pragma unroll
     for (int k = 0; k < DIMS; ++k) {
         if (NARROW == TAY_NARROW_PP) {
-            float d = a_lo[k] - b_lo[k];
-            ok = ok && !(d < -r[k] || d > r[k]);
+            ok = ok & !(fabsf(a_lo[k] - b_lo[k]) > r[k]);
         } else if (NARROW == TAY_NARROW_PN) {
-            ok = ok && !(a_lo[k] - b_hi[k] > r[k]) && !(b_lo[k] - a_lo[k] > r[k]);
+            ok = ok & !(a_lo[k] - b_hi[k] > r[k]) & !(b_lo[k] - a_lo[k] > r[k]);
         } else if (NARROW == TAY_NARROW_NP) {
-            ok = ok && !(a_lo[k] - b_lo[k] > r[k]) && !(b_lo[k] - a_hi[k] > r[k]);
+            ok = ok & !(a_lo[k] - b_lo[k] > r[k]) & !(b_lo[k] - a_hi[k] > r[k]);
         } else {
-            ok = ok && !(a_lo[k] - b_hi[k] > r[k]) && !(b_lo[k] - a_hi[k] > r[k]);
+            ok = ok & !(a_lo[k] - b_hi[k] > r[k]) & !(b_lo[k] - a_hi[k] > r[k]);
         }
     }
     return ok;
 }
 
+/* Orders this wavefront's LDS accesses: everything before is complete and
+ * visible to every lane of the wavefront before anything after starts. */
+__device__ __forceinline__ void tay_wave_sync(void) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* The seen candidates of a tile are consumed in chunks of 64.  Filter: each
+ * chunk's boxes are staged in LDS and every lane box-tests all of them
+ * (broadcast reads), setting bit c of its 64-bit mask for each passing
+ * candidate c.  Compact: the wavefront then loops over the set bits, each
+ * lane taking its lowest remaining bit, so the model body runs with nearly
+ * all lanes busy instead of once per candidate under a sparse lane mask.
+ * Bits are taken in ascending order and chunks follow the candidate order,
+ * so every seer still meets its pairs in the reference's sweep order. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
           bool COUNT, class Call>
 __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
@@ -99,9 +120,15 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
     constexpr bool SEER_BOX = NARROW == TAY_NARROW_NP || NARROW == TAY_NARROW_NN;
     constexpr bool SEEN_BOX = NARROW == TAY_NARROW_PN || NARROW == TAY_NARROW_NN;
+    constexpr unsigned WAVES = TAY_SEE_BLOCK / 64u;
+
+    __shared__ float chunk_lo[WAVES][64][4];
+    __shared__ float chunk_hi[SEEN_BOX ? WAVES : 1u][64][4];
+    __shared__ unsigned chunk_idx[WAVES][64];
 
     const unsigned lane = threadIdx.x & 63u;
-    const unsigned tile = tay_xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const unsigned wave = threadIdx.x >> 6;
+    const unsigned tile = tay_xcd_block() * WAVES + wave;
     if (tile >= args.tiles)
         return;
     const unsigned i0 = tile * 64u;
@@ -128,55 +155,134 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     }
     typename T::context *ctx = (typename T::context *)args.context;
     unsigned pairs = 0;
+    bool member = active;   /* lane belongs to the lane group being processed */
+    unsigned fill = 0;      /* slots used in the current chunk (uniform) */
+    unsigned cand = 0;      /* lane k: storage index of the chunk's slot k */
 
-    auto visit = [&](unsigned beg, unsigned end) {
-        for (unsigned j = beg; j < end; ++j) {
-            union {
-                typename T::seen agent;
-                unsigned w[SEEN_WORDS];
-            } B;
+    /* box-test candidates [c0, c1) of the staged chunk into a 32-bit mask */
+    auto filter = [&](unsigned c0, unsigned c1) {
+        unsigned bits = 0;
+        for (unsigned c = c0; c < c1; ++c) {
+            const float *b_lo = chunk_lo[wave][c];
+            const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
+            bool ok = member & tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
+            if (SKIP_SELF)
+                ok = ok & (chunk_idx[wave][c] != i);
+            bits |= ok ? 1u << (c - c0) : 0u;
+        }
+        return bits;
+    };
+
+    auto process = [&]() {
+        if (lane < fill) {
 #pragma unroll
-            for (unsigned k = 0; k < SEEN_WORDS; ++k)
-                B.w[k] = args.seen_cols[k * args.seen_cap + j];
-            float b_lo[3] = {0.0f, 0.0f, 0.0f}, b_hi[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int k = 0; k < DIMS; ++k) {
-                b_lo[k] = __uint_as_float(B.w[2 + k]);
-                b_hi[k] = SEEN_BOX ? __uint_as_float(B.w[6 + k]) : b_lo[k];
+            for (int k = 0; k < 3; ++k) {
+                chunk_lo[wave][lane][k] = k < DIMS ? __uint_as_float(args.seen_cols[(2u + k) * args.seen_cap + cand]) : 0.0f;
+                if (SEEN_BOX)
+                    chunk_hi[wave][lane][k] =
+                        k < DIMS ? __uint_as_float(args.seen_cols[(6u + k) * args.seen_cap + cand]) : 0.0f;
             }
-            bool ok = active && (!SKIP_SELF || j != i) && tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
-            if (ok) {
+            chunk_idx[wave][lane] = cand;
+        }
+        tay_wave_sync();
+
+        const unsigned lo_bits = filter(0u, fill < 32u ? fill : 32u);
+        const unsigned hi_bits = fill > 32u ? filter(32u, fill) : 0u;
+        unsigned long long mask = ((unsigned long long)hi_bits << 32) | lo_bits;
+        if (COUNT)
+            pairs += (unsigned)__popcll(mask);
+
+        for (;;) {
+            const bool more = mask != 0ull;
+            if (!__any(more))
+                break;
+            if (more) {
+                const unsigned c = (unsigned)__builtin_ctzll(mask);
+                mask &= mask - 1ull;
+                const unsigned j = chunk_idx[wave][c];
+                union {
+                    typename T::seen agent;
+                    unsigned w[SEEN_WORDS];
+                } B;
+#pragma unroll
+                for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                    B.w[k] = args.seen_cols[k * args.seen_cap + j];
+#pragma unroll
+                for (int k = 0; k < DIMS; ++k)
+                    B.w[2 + k] = __float_as_uint(chunk_lo[wave][c][k]);
                 call(&A.agent, &B.agent, ctx);
-                if (COUNT)
-                    ++pairs;
             }
+        }
+        tay_wave_sync();
+        fill = 0;
+    };
+
+    auto add_range = [&](unsigned beg, unsigned end) {
+        while (beg < end) {
+            const unsigned room = 64u - fill;
+            const unsigned take = end - beg < room ? end - beg : room;
+            if (lane - fill < take)
+                cand = beg + (lane - fill);
+            fill += take;
+            beg += take;
+            if (fill == 64u)
+                process();
         }
     };
 
     if (SEEN_GRID) {
+        /* A group of lanes whose bounding box would span more than `limit`
+         * cells (about one cell plus the radius on every side) is split in
+         * halves, recursively: aligned power-of-two lane ranges visited
+         * depth first.  Seers far apart in space then stop scanning each
+         * other's neighbourhoods; each seer still sees its cells in x->y->z
+         * order. */
         const TayGridGeom *g = args.geom;
-        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        unsigned limit = 1;
 #pragma unroll
-        for (int k = 0; k < DIMS; ++k) {
-            float tmin = tay_wave_min(active ? a_lo[k] : __builtin_inff());
-            float tmax = tay_wave_max(active ? a_hi[k] : -__builtin_inff());
-            /* margin (|p| + r) * 2^-20 covers the rounding of p - r and of
-             * the box test, so no cell holding a passing pair is missed */
-            float mlo = (fabsf(tmin) + r[k]) * 0x1p-20f;
-            float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
-            lo[k] = tay_cell_axis((tmin - r[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
-            hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
-        }
+        for (int k = 0; k < DIMS; ++k)
+            limit *= (unsigned)ceilf(2.0f * r[k] / g->cell_size[k]) + 3u;
         const unsigned cx = (unsigned)g->count[0];
         const unsigned cxy = cx * (unsigned)g->count[1];
-        for (int x = lo[0]; x <= hi[0]; ++x)
-            for (int y = lo[1]; y <= hi[1]; ++y)
-                for (int z = lo[2]; z <= hi[2]; ++z) {
-                    const TayCellRange cr = args.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
-                    visit(cr.beg, cr.end);
-                }
+        const unsigned n_active = args.seer_count - i0 < 64u ? args.seer_count - i0 : 64u;
+        unsigned beg = 0, width = 64;
+        while (beg < n_active) {
+            member = active && lane - beg < width;
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            unsigned cells = 1;
+#pragma unroll
+            for (int k = 0; k < DIMS; ++k) {
+                float tmin = tay_wave_min(member ? a_lo[k] : __builtin_inff());
+                float tmax = tay_wave_max(member ? a_hi[k] : -__builtin_inff());
+                /* margin (|p| + r) * 2^-20 covers the rounding of p - r and
+                 * of the box test, so no cell holding a passing pair is
+                 * missed */
+                float mlo = (fabsf(tmin) + r[k]) * 0x1p-20f;
+                float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
+                lo[k] = tay_cell_axis((tmin - r[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
+                hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
+                cells *= (unsigned)(hi[k] - lo[k] + 1);
+            }
+            if (cells > limit && width > 1u) {
+                width >>= 1;
+                continue;
+            }
+            for (int x = lo[0]; x <= hi[0]; ++x)
+                for (int y = lo[1]; y <= hi[1]; ++y)
+                    for (int z = lo[2]; z <= hi[2]; ++z) {
+                        const TayCellRange cr = args.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                        add_range(cr.beg, cr.end);
+                    }
+            if (fill)
+                process();
+            beg += width;
+            while (width < 64u && (beg & (2u * width - 1u)) == 0u)
+                width <<= 1;
+        }
     } else {
-        visit(0u, args.seen_count);
+        add_range(0u, args.seen_count);
+        if (fill)
+            process();
     }
 
     if (active) {

@@ -77,6 +77,7 @@ struct TayState {
     TayGpuProgram *program;
     int count_pairs;
     int timing;
+    int fetch_mode;
 };
 
 namespace {
@@ -247,6 +248,7 @@ TAY_API TayState *tay_create_state() {
     state->program = nullptr;
     state->count_pairs = 0;
     state->timing = 0;
+    state->fetch_mode = TAY_FETCH_EVERY_RUN;
     return state;
 }
 
@@ -479,11 +481,8 @@ TAY_API int tay_run(TayState *state, int steps) {
             }
     }
 
-    for (TayGroup &g : state->groups)
-        if (g.active && !tay_gpu_space_fetch(g.gpu, g.storage, g.count)) {
-            set_device_error(state, "fetch agents");
-            return 0;
-        }
+    if (state->fetch_mode == TAY_FETCH_EVERY_RUN && !tay_fetch_agents(state, nullptr))
+        return 0;
     if (!tay_gpu_synchronize()) {
         set_device_error(state, "run");
         return 0;
@@ -577,4 +576,21 @@ TAY_API int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, 
 TAY_API TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group) {
     (void)state;
     return group->gpu;
+}
+
+TAY_API void tay_set_fetch_mode(TayState *state, int mode) {
+    state->fetch_mode = mode == TAY_FETCH_ON_DEMAND ? TAY_FETCH_ON_DEMAND : TAY_FETCH_EVERY_RUN;
+}
+
+/* tay/ocl.c:261-272, for one group or all */
+TAY_API int tay_fetch_agents(TayState *state, TayGroup *group) {
+    for (TayGroup &g : state->groups) {
+        if (!g.active || !g.gpu || (group && group != &g))
+            continue;
+        if (!tay_gpu_space_fetch(g.gpu, g.storage, g.count)) {
+            set_device_error(state, "fetch agents");
+            return 0;
+        }
+    }
+    return 1;
 }
