@@ -149,11 +149,6 @@ int state_compile(TayState *state) {
             return 0;
         }
     }
-    if (!state->device_enabled && !tay_gpu_available()) {
-        set_error(state, TAY_ERROR_OCL, "device not found");
-        return 0;
-    }
-    state->device_enabled = 1;
 
     for (unsigned i = 0; i < state->passes_count; ++i) {
         TayPass &p = state->passes[i];
@@ -178,6 +173,12 @@ int state_compile(TayState *state) {
             return 0;
         }
     }
+
+    if (!state->device_enabled && !tay_gpu_available()) {
+        set_error(state, TAY_ERROR_OCL, "device not found");
+        return 0;
+    }
+    state->device_enabled = 1;
 
     release_device(state);
     for (TayGroup &g : state->groups) {

@@ -1,0 +1,199 @@
+"""Checks that pin the oracle where the reference's fixtures leave it open:
+independent statements of the Morton code and the grid geometry, brute-force
+pair sets, an FP64 recomputation of the float sums, and thread-count
+independence.  Every check uses a different algorithm from the oracle's
+(bit loops instead of mask tricks, numpy float32 arithmetic, all-pairs
+search), so a dropped term or a wrong index fails one of them."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from tay_amd import agents as A
+from tay_amd import api
+from tay_amd import workloads as W
+
+from conftest import GOLDEN
+
+
+def interleave(x, y, z, dims):
+    """Morton key by an explicit bit loop: bit b of axis a goes to bit
+    b * dims + a."""
+    key = 0
+    for b in range(32 // dims if dims > 1 else 32):
+        for a, v in enumerate((x, y, z)[:dims]):
+            key |= ((v >> b) & 1) << (b * dims + a)
+    return key & 0xFFFFFFFF
+
+
+def test_morton_3d_against_bit_loop(oracle):
+    g = np.random.default_rng(0)
+    samples = [(0, 0, 0), (1023, 1023, 1023), (1, 0, 0), (0, 1, 0), (0, 0, 1), (512, 7, 1000)]
+    samples += [tuple(int(t) for t in g.integers(0, 1024, 3)) for _ in range(3000)]
+    for x, y, z in samples:
+        key = oracle.oracle_morton_key(x, y, z, 3)
+        assert key == interleave(x, y, z, 3)
+        xs, ys, zs = C.c_int(), C.c_int(), C.c_int()
+        oracle.oracle_morton_decode(key, 3, C.byref(xs), C.byref(ys), C.byref(zs))
+        assert (xs.value, ys.value, zs.value) == (x, y, z)
+
+
+def test_morton_2d_1d_4d(oracle):
+    g = np.random.default_rng(1)
+    for _ in range(2000):
+        x, y = (int(t) for t in g.integers(0, 65536, 2))
+        assert oracle.oracle_morton_key(x, y, 0, 2) == interleave(x, y, 0, 2)
+        assert oracle.oracle_morton_key(x, y, 0, 1) == x
+        assert oracle.oracle_morton_key(x, y, 3, 4) == 0   # 4D is not implemented in the reference
+
+
+def test_morton_is_monotone_along_each_axis(oracle):
+    for axis in range(3):
+        prev = -1
+        for t in range(1024):
+            c = [5, 9, 13]
+            c[axis] = t
+            key = oracle.oracle_morton_key(c[0], c[1], c[2], 3)
+            assert key > prev
+            prev = key
+
+
+def _reference_fixture():
+    return np.load(os.path.join(GOLDEN, "ref_test_model_clump.npz"))
+
+
+def test_grid_geometry_closed_form_in_float32():
+    """count = ceil(size / part), cell = size / count with
+    size = (max - min) + part * 0.001, evaluated in numpy float32, equals the
+    reference CpuGrid's geometry of step 1."""
+    d = _reference_fixture()
+    p = d["p"][:, :3].astype(np.float32)
+    part = np.float32(d["part"])
+    lo, hi = p.min(0), p.max(0)
+    size = (hi - lo) + part * np.float32(0.001)
+    count = np.ceil(size / part).astype(np.int32)
+    cell = size / count.astype(np.float32)
+    assert np.array_equal(lo.view(np.uint32), d["grid_origin"][0].view(np.uint32))
+    assert np.array_equal(count, d["grid_cell_count"][0])
+    assert np.array_equal(cell.view(np.uint32), d["grid_cell_size"][0].view(np.uint32))
+
+
+def test_reference_keys_are_morton_of_cell_indices():
+    """The reference CpuZGrid's part_i of step 1 is the 10-bit Morton code of
+    floor((p - origin) / cell); CpuGrid's is the row-major id."""
+    d = _reference_fixture()
+    p = d["p"][:, :3].astype(np.float32)
+    for structure in ("zgrid", "grid"):
+        origin = d[structure + "_origin"][0]
+        cell = d[structure + "_cell_size"][0]
+        count = d[structure + "_cell_count"][0]
+        idx = np.floor((p - origin) / cell).astype(np.int64)
+        order = d[structure + "_order"][0]
+        if structure == "zgrid":
+            expect = [interleave(int(a), int(b), int(c), 3) for a, b, c in idx[order]]
+        else:
+            expect = list(idx[order, 0] + idx[order, 1] * count[0] + idx[order, 2] * count[0] * count[1])
+        assert list(d[structure + "_part_i"][0]) == expect
+
+
+def test_max_position_agent_stays_in_last_cell(oracle, models):
+    recs = W.plain_agents(500, seed=3, box=333.0)
+    recs["p"][7, :3] = [333.0, 333.0, 333.0]   # the box maximum on every axis
+    with W.Simulation(oracle, W.plain_model(20.0), recs, api.TAY_GPU_Z_GRID, funcs=models) as s:
+        s.run(1)
+        a = s.agents()
+        origin, size, count, _ = oracle.grid_info(s.state, s.group)
+    idx = np.floor((a["p"][:, :3] - a["v"][:, :3] - origin[:3]) / size[:3])
+    assert idx.max() <= count[:3].max() - 1
+
+
+def _brute_force_counts(p, r):
+    """Pairs passing the inclusive per-axis box test, self excluded, by
+    all-pairs float32 arithmetic."""
+    d = p[:, None, :] - p[None, :, :]
+    ok = np.all(~(np.abs(d) > np.float32(r)), axis=2)
+    np.fill_diagonal(ok, False)
+    return ok
+
+
+@pytest.mark.parametrize("space", [api.TAY_CPU_SIMPLE, api.TAY_CPU_GRID, api.TAY_CPU_Z_GRID, api.TAY_GPU_Z_GRID])
+def test_pair_counts_against_all_pairs(oracle, models, space):
+    recs = W.test_agents(1500, seed=4, box=1000.0, clump=0.1)
+    r = 70.0
+    model = W.test_model(r)
+    model.passes = model.passes[:1]
+    with W.Simulation(oracle, model, recs, space, funcs=models) as s:
+        s.run(1)
+        a = s.agents()
+        _, narrow = oracle.pair_counts(s.state, 0)
+    a = a[np.argsort(a["result_index"])]
+    ok = _brute_force_counts(recs["p"][:, :3].astype(np.float32), r)
+    assert narrow == int(ok.sum())
+    assert np.array_equal(a["b_buffer_count"], ok.sum(1))
+    assert int(ok.sum()) % 2 == 0
+
+
+def test_plain_sums_against_fp64(oracle, models):
+    """Each float32 neighbour-position sum is within 1e-5 of the sum of
+    |terms| of an FP64 recomputation over the brute-force pair set."""
+    recs = W.plain_agents(1200, seed=5, box=500.0)
+    r = 40.0
+    with W.Simulation(oracle, W.plain_model(r), recs, api.TAY_GPU_Z_GRID, funcs=models) as s:
+        s.run(1)
+        a = s.agents()
+    a = a[np.argsort(a["result_index"])]
+    p32 = recs["p"][:, :3].astype(np.float32)
+    d = p32[:, None, :] - p32[None, :, :]
+    d2 = (d * d).sum(axis=2, dtype=np.float32)
+    ok = (d2 <= np.float32(r) * np.float32(r))
+    np.fill_diagonal(ok, False)
+    p64 = p32.astype(np.float64)
+    s64 = ok.astype(np.float64) @ p64
+    mag = ok.astype(np.float64) @ np.abs(p64)
+    assert np.array_equal(a["last_ncount"], ok.sum(1))
+    err = np.abs(a["last_nsum"][:, :3].astype(np.float64) - s64)
+    assert np.all(err <= 1e-5 * np.maximum(mag, 1e-30))
+
+
+def test_results_do_not_depend_on_thread_count(models):
+    """The oracle library is one process-wide pool: compare a 1-thread run in
+    a fresh oracle instance with the session's multi-threaded one."""
+    import subprocess
+    import sys
+    code = r'''
+import sys, ctypes, numpy as np
+sys.path.insert(0, %r)
+from tay_amd import api, workloads as W
+ora = api.TayLib(%r); models = ctypes.CDLL(%r)
+ora.tay_threads_start(int(sys.argv[1]), 0)
+recs = W.test_agents(3000, seed=6, clump=0.2)
+with W.Simulation(ora, W.test_model(50.0), recs, api.TAY_CPU_GRID, funcs=models) as s:
+    s.run(3)
+    sys.stdout.buffer.write(s.agents().tobytes())
+''' % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+       os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_build", "liboracle.so"),
+       os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_build", "libmodels.so"))
+    one = subprocess.run([sys.executable, "-c", code, "1"], capture_output=True, check=True).stdout
+    many = subprocess.run([sys.executable, "-c", code, "7"], capture_output=True, check=True).stdout
+    assert len(one) == 3000 * A.AGENT.itemsize and one == many
+
+
+def test_memory_cap_coarsening(oracle, models):
+    """With a 1 MB cell budget (131072 cells) the GPU Z-grid statement picks
+    the smallest integer k whose ceil(count / k) fits, found here by search."""
+    recs = W.plain_agents(4000, seed=7, box=1000.0)
+    part = 10.0
+    with W.Simulation(oracle, W.plain_model(5.0), recs, api.TAY_GPU_Z_GRID, funcs=models, shared_mb=1,
+                      part=part) as s:
+        s.run(1)
+        origin, size, count, cells = oracle.grid_info(s.state, s.group)
+    p = recs["p"][:, :3].astype(np.float32)
+    sz = (p.max(0) - p.min(0)) + np.float32(part) * np.float32(0.001)
+    base = np.ceil(sz / np.float32(part)).astype(np.int64)
+    max_cells = (1 << 20) // 8
+    k = next(k for k in range(1, 10000)
+             if np.all(-(-base // k) <= 1024) and np.prod(-(-base // k)) <= max_cells)
+    assert k > 1
+    assert list(count[:3]) == list(-(-base // k)) and cells == np.prod(-(-base // k))
+    assert np.array_equal(size[:3].view(np.uint32), (sz / count[:3].astype(np.float32)).view(np.uint32))
