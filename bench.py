@@ -82,9 +82,13 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--agents", type=int, default=1_000_000)
+    ap.add_argument("--cell-size", type=float, default=25.0, help="Z-grid cell size (min_part_sizes)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one plain device-to-device copy of MIB MiB (reads and writes MIB MiB), the "
+                         "calibration kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -103,14 +107,14 @@ def main():
     model = W.flock_model(radius=25.0, weights=(1.5, 1.0, 1.0), dt=0.1, max_speed=2.0)
 
     # counting twin: same inputs, same trajectory, pair counts of the timed steps
-    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as twin:
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as twin:
         lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
         lib.tay_set_pair_counting(twin.state, 1)
         twin.run(args.warmup) if args.warmup else None
         twin.run(args.steps)
         p_see = lib.pair_counts(twin.state, 0)[1]
 
-    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as sim:
         lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
         if args.warmup:
             sim.run(args.warmup)
@@ -145,7 +149,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "flocking-1M", "agents": args.agents, "clusters": 64, "sigma": 30.0,
-                   "box": 2000.0, "radius": 25.0, "cell_size": 25.0, "weights": [1.5, 1.0, 1.0], "dt": 0.1,
+                   "box": 2000.0, "radius": 25.0, "cell_size": args.cell_size, "weights": [1.5, 1.0, 1.0], "dt": 0.1,
                    "max_speed": 2.0, "structure": "TAY_GPU_Z_GRID"},
         "steps_per_s": steps_per_s,
         "p_see_per_step": p_see / args.steps,
@@ -167,6 +171,12 @@ def main():
         "roofline_hbm": {"achieved": hbm_achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": hbm_achieved / PEAK_HBM_GBS, "bytes_per_agent": B_SEE_PER_AGENT},
     }
+    if args.copy_probe:
+        probe_ms = ctypes.c_double()
+        if not lib.tay_gpu_copy_probe(args.copy_probe << 20, ctypes.byref(probe_ms)):
+            raise SystemExit("copy probe: " + lib.tay_gpu_last_error().decode())
+        out["copy_probe"] = {"bytes": args.copy_probe << 20, "ms": probe_ms.value,
+                             "GB/s": 2 * (args.copy_probe << 20) / probe_ms.value / 1e6}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(recs, model, args.cpu_threads, args.cpu_steps)
     print(json.dumps(out))

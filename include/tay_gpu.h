@@ -85,6 +85,12 @@ int tay_gpu_space_sort(TayGpuSpace *space);
 int tay_gpu_space_grid_info(TayGpuSpace *space, float *origin, float *cell_sizes, int *cell_counts,
                             unsigned long long *cells_count);
 
+/* Diagnostic: one streaming copy of `bytes` between two fresh device buffers
+ * (reads and writes `bytes`), timed with HIP events.  The calibration kernel
+ * for HBM traffic counters and a measured copy bandwidth; no reference
+ * counterpart. */
+int tay_gpu_copy_probe(unsigned long long bytes, double *ms);
+
 /* ------------------------------------------------------------ programs */
 
 typedef struct TayGpuPassDesc {
@@ -119,6 +125,7 @@ typedef struct TayGpuPassShape {
     int seen_kind;              /* TayGpuSpaceKind */
     int dims;
     int skip_self;              /* 1 when seer and seen are one group and self_see is 0 */
+    unsigned context_size;      /* bytes of the pass context */
 } TayGpuPassShape;
 
 /* Composes and compiles a program to a gfx950 code object without loading

@@ -57,7 +57,7 @@ class TayGpuPassShape(C.Structure):
     _fields_ = [("type", C.c_int), ("func_name", C.c_char_p),
                 ("seer_agent_size", C.c_uint), ("seen_agent_size", C.c_uint),
                 ("seer_is_point", C.c_int), ("seen_is_point", C.c_int),
-                ("seen_kind", C.c_int), ("dims", C.c_int), ("skip_self", C.c_int)]
+                ("seen_kind", C.c_int), ("dims", C.c_int), ("skip_self", C.c_int), ("context_size", C.c_uint)]
 
 
 _vp = C.c_void_p
@@ -103,6 +103,7 @@ _GPU = {
     "tay_gpu_available": (_i, []),
     "tay_gpu_last_error": (C.c_char_p, []),
     "tay_gpu_synchronize": (_i, []),
+    "tay_gpu_copy_probe": (_i, [C.c_ulonglong, C.POINTER(C.c_double)]),
     "tay_gpu_space_create": (_vp, [_i, _u, _u, _i, _i, C.POINTER(C.c_float), C.c_ulonglong]),
     "tay_gpu_space_destroy": (None, [_vp]),
     "tay_gpu_space_fix_box": (_i, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),

@@ -40,19 +40,20 @@ HOST_ONLY = {"tay_api.cpp"}
 EMBEDDED = {"device_abi_text": "device_abi.h", "jit_device_text": "jit_device.h"}
 
 
-def _write_embedded():
-    """Embed device_abi.h and jit_device.h as string constants."""
-    os.makedirs(BUILD, exist_ok=True)
+def _write_embedded(overrides=None, build_dir=BUILD):
+    """Embed device_abi.h and jit_device.h as string constants (overrides
+    maps a header name to another file to embed in its place)."""
+    os.makedirs(build_dir, exist_ok=True)
     out = ['// generated by tay_amd/build.py from csrc/device_abi.h and csrc/jit_device.h',
            '#include "internal.h"', 'namespace tay {']
     for name, header in EMBEDDED.items():
-        with open(os.path.join(CSRC, header)) as f:
+        with open((overrides or {}).get(header, os.path.join(CSRC, header))) as f:
             text = f.read()
         if ")TAYEMBED\"" in text:
             raise RuntimeError(header + " contains the raw-string delimiter")
         out.append('extern const char *const %s = R"TAYEMBED(%s)TAYEMBED";' % (name, text))
     out.append("}  // namespace tay")
-    path = os.path.join(BUILD, "embedded.cpp")
+    path = os.path.join(build_dir, "embedded.cpp")
     content = "\n".join(out) + "\n"
     if not os.path.exists(path) or open(path).read() != content:
         with open(path, "w") as f:
@@ -67,14 +68,15 @@ def _run(cmd, cwd=None):
     return proc.stdout
 
 
-def build_library(verbose=False):
-    """Compile every HIP/C++ source for gfx950 and link libtay_amd.so."""
-    embedded = _write_embedded()
+def build_library(verbose=False, out=LIB, overrides=None, build_dir=BUILD):
+    """Compile every HIP/C++ source for gfx950 and link libtay_amd.so (or
+    `out`, embedding the `overrides` headers, for experiments)."""
+    embedded = _write_embedded(overrides, build_dir)
     objs = []
     jobs = []
     for src in SOURCES + [embedded]:
         path = src if os.path.isabs(src) else os.path.join(CSRC, src)
-        obj = os.path.join(BUILD, os.path.basename(path) + ".o")
+        obj = os.path.join(build_dir, os.path.basename(path) + ".o")
         objs.append(obj)
         if src in HOST_ONLY:
             # host C++ without HIP headers (sees Tay's float4, not HIP's)
@@ -87,11 +89,11 @@ def build_library(verbose=False):
         for o in outs:
             if o.strip():
                 print(o)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
          ["-Wl,-Bsymbolic", "-lhiprtc", "-lamdhip64"])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_oracle(with_reference=True):

@@ -541,6 +541,15 @@ __global__ __launch_bounds__(kBuildBlock) void k_soa_to_aos(const unsigned *__re
     }
 }
 
+/* Streaming copy of n 16-byte vectors: the calibration kernel for HBM
+ * counters and a measured copy bandwidth. */
+__global__ __launch_bounds__(kBuildBlock) void k_copy_probe(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                            unsigned long long n) {
+    for (unsigned long long i = blockIdx.x * (unsigned long long)kBuildBlock + threadIdx.x; i < n;
+         i += (unsigned long long)gridDim.x * kBuildBlock)
+        dst[i] = src[i];
+}
+
 unsigned blocks_for(unsigned long long work, unsigned per_block, unsigned cap_blocks) {
     unsigned long long b = (work + per_block - 1) / per_block;
     if (b == 0)
@@ -771,6 +780,35 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
         return 0;
     sp->cur ^= 1;
     return 1;
+}
+
+TAY_API int tay_gpu_copy_probe(unsigned long long bytes, double *ms) {
+    hipStream_t s = stream();
+    if (!s)
+        return 0;
+    bytes &= ~15ull;
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t beg = nullptr, end = nullptr;
+    float t = 0.0f;
+    bool ok = device_alloc(&src, bytes, "probe source") && device_alloc(&dst, bytes, "probe destination") &&
+              hip_ok(hipMemsetAsync(src, 0, bytes, s), "hipMemsetAsync") &&
+              hip_ok(hipEventCreate(&beg), "hipEventCreate") && hip_ok(hipEventCreate(&end), "hipEventCreate") &&
+              hip_ok(hipEventRecord(beg, s), "hipEventRecord");
+    if (ok) {
+        k_copy_probe<<<2048, kBuildBlock, 0, s>>>((const uint4 *)src, (uint4 *)dst, bytes / 16);
+        ok = launch_ok("copy probe") && hip_ok(hipEventRecord(end, s), "hipEventRecord") &&
+             hip_ok(hipEventSynchronize(end), "hipEventSynchronize") &&
+             hip_ok(hipEventElapsedTime(&t, beg, end), "hipEventElapsedTime");
+    }
+    if (beg)
+        (void)hipEventDestroy(beg);
+    if (end)
+        (void)hipEventDestroy(end);
+    device_free(src);
+    device_free(dst);
+    if (ms)
+        *ms = t;
+    return ok;
 }
 
 TAY_API int tay_gpu_space_grid_info(TayGpuSpace *sp, float *origin, float *cell_sizes, int *cell_counts,

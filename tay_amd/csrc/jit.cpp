@@ -117,6 +117,7 @@ TayGpuPassShape shape_of(const TayGpuPassDesc &d) {
     sh.seer_agent_size = d.seer->agent_size;
     sh.seer_is_point = d.seer->is_point;
     sh.dims = d.seer->dims;
+    sh.context_size = d.context_size;
     if (d.type == TAY_GPU_PASS_SEE) {
         sh.seen_agent_size = d.seen->agent_size;
         sh.seen_is_point = d.seen->is_point;
@@ -140,26 +141,36 @@ std::string compose(const char *const *sources, unsigned sources_count, const Ta
     text += "#line 1 \"jit_device.h\"\n";
     text += jit_device_text;
     text += "\n#line 1 \"tay_passes\"\n";
+    /* tuning: TAY_SEE_WAVES_PER_EU=<n> asks the compiler to fit n wavefronts
+     * per SIMD into the see kernels' register budget */
+    std::string occupancy;
+    if (const char *w = std::getenv("TAY_SEE_WAVES_PER_EU")) {
+        const int n = std::atoi(w);
+        if (n >= 1 && n <= 8)
+            occupancy = " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
+    }
     for (unsigned i = 0; i < shapes_count; ++i) {
         const TayGpuPassShape &p = shapes[i];
         if (p.type == TAY_GPU_PASS_SEE) {
+            append(text, "extern \"C\" __global__ __launch_bounds__(%u)%s", kSeeBlock, occupancy.c_str());
             append(text,
-                   "extern \"C\" __global__ __launch_bounds__(%u) void tay_pass_%u(TaySeeArgs args) {\n"
+                   " void tay_pass_%u(TaySeeArgs args) {\n"
                    "    typedef tay_see_traits<decltype(%s)> T;\n"
-                   "    tay_see_tiles<T, %uu, %uu, %d, %s, %s, %d, %s>(args,\n"
+                   "    tay_see_tiles<T, %uu, %uu, %d, %s, %s, %d, %s, %uu>(args,\n"
                    "        [](T::seer *a, T::seen *b, T::context *c) { %s(a, b, c); });\n"
                    "}\n",
-                   kSeeBlock, i, p.func_name, p.seer_agent_size / 4u, p.seen_agent_size / 4u,
+                   i, p.func_name, p.seer_agent_size / 4u, p.seen_agent_size / 4u,
                    narrow_kind(p.seer_is_point, p.seen_is_point), p.skip_self ? "true" : "false",
                    p.seen_kind == TAY_GPU_KIND_Z_GRID ? "true" : "false", p.dims, count_pairs ? "true" : "false",
+                   (p.context_size + 3u) / 4u,
                    p.func_name);
         } else {
             append(text,
                    "extern \"C\" __global__ __launch_bounds__(%u) void tay_pass_%u(TayActArgs args) {\n"
                    "    typedef tay_act_traits<decltype(%s)> T;\n"
-                   "    tay_act_agents<T, %uu>(args, [](T::agent *a, T::context *c) { %s(a, c); });\n"
+                   "    tay_act_agents<T, %uu, %uu>(args, [](T::agent *a, T::context *c) { %s(a, c); });\n"
                    "}\n",
-                   kActBlock, i, p.func_name, p.seer_agent_size / 4u, p.func_name);
+                   kActBlock, i, p.func_name, p.seer_agent_size / 4u, (p.context_size + 3u) / 4u, p.func_name);
         }
     }
     return text;
@@ -319,7 +330,8 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
         pass.launches = 0;
         char name[64];
         std::snprintf(name, sizeof(name), "tay_pass_%u", i);
-        const unsigned ctx_bytes = passes[i].context_size ? passes[i].context_size : 4u;
+        /* whole words: kernels copy the context word by word */
+        const unsigned ctx_bytes = passes[i].context_size ? (passes[i].context_size + 3u) & ~3u : 4u;
         bool ok = hip_ok(hipModuleGetFunction(&pass.kernel, p->module, name), "hipModuleGetFunction") &&
                   device_alloc(&pass.context, ctx_bytes, "pass context") &&
                   hip_ok(hipHostMalloc(&pass.context_host, ctx_bytes, hipHostMallocDefault), "hipHostMalloc") &&

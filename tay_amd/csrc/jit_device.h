@@ -13,10 +13,9 @@
  * and a rounding margin, selects the seen cells; the wavefront walks them x
  * outermost, then y, then z, and each cell's agents in storage order, so every
  * seer meets its pairs in the order of the reference's grid sweep
- * (tay/cpu_grid.c:222-261) and accumulates exactly the same float sums.  The
- * seen index is uniform across the wavefront, so seen words arrive through
- * scalar loads and every lane applies the reference's per-axis box test
- * (tay/space.c:62-104) to its own seer.
+ * (tay/cpu_grid.c:222-261) and accumulates exactly the same float sums.
+ * Every lane applies the reference's per-axis box test (tay/space.c:62-104)
+ * to its own seer.
  */
 
 template <typename F> struct tay_see_traits;
@@ -33,6 +32,36 @@ template <typename A, typename C> struct tay_act_traits<void(A *, C *)> {
 };
 
 enum { TAY_NARROW_PP = 0, TAY_NARROW_PN = 1, TAY_NARROW_NP = 2, TAY_NARROW_NN = 3 };
+
+template <class C> struct tay_is_void { static constexpr bool value = false; };
+template <> struct tay_is_void<void> { static constexpr bool value = true; };
+template <class C, unsigned WORDS> struct tay_ctx_fits {
+    static constexpr bool value = WORDS > 0u && WORDS <= 64u && sizeof(C) <= WORDS * 4u;
+};
+template <unsigned WORDS> struct tay_ctx_fits<void, WORDS> { static constexpr bool value = false; };
+
+/* A pass context of up to 64 words is copied once per kernel into
+ * registers with aligned uniform loads.  Model contexts are packed structs
+ * (alignment 1), so reading their fields from memory inside the pair loop
+ * would cost a vector load per field per pair. */
+template <class C, unsigned WORDS, bool COPY = tay_ctx_fits<C, WORDS>::value> struct tay_ctx_holder {
+    C *p;
+    __device__ __forceinline__ explicit tay_ctx_holder(const void *q) : p((C *)q) {}
+    __device__ __forceinline__ C *get() { return p; }
+};
+template <class C, unsigned WORDS> struct tay_ctx_holder<C, WORDS, true> {
+    union {
+        C ctx;
+        unsigned w[WORDS];
+    } u;
+    __device__ __forceinline__ explicit tay_ctx_holder(const void *q) {
+        const unsigned *words = (const unsigned *)q;
+#pragma unroll
+        for (unsigned k = 0; k < WORDS; ++k)
+            u.w[k] = words[k];
+    }
+    __device__ __forceinline__ C *get() { return &u.ctx; }
+};
 
 #define TAY_SEE_BLOCK 256u
 
@@ -114,7 +143,7 @@ __device__ __forceinline__ void tay_wave_sync(void) {
  * Bits are taken in ascending order and chunks follow the candidate order,
  * so every seer still meets its pairs in the reference's sweep order. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
-          bool COUNT, class Call>
+          bool COUNT, unsigned CTX_WORDS, class Call>
 __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
     static_assert(sizeof(typename T::seer) <= SEER_WORDS * 4u, "seer record larger than the group's agent size");
     static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
@@ -153,7 +182,8 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
         a_hi[k] = SEER_BOX ? __uint_as_float(A.w[6 + k]) : a_lo[k];
         r[k] = args.radii[k];
     }
-    typename T::context *ctx = (typename T::context *)args.context;
+    tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(args.context);
+    typename T::context *ctx = ctx_holder.get();
     unsigned pairs = 0;
     bool member = active;   /* lane belongs to the lane group being processed */
     unsigned fill = 0;      /* slots used in the current chunk (uniform) */
@@ -162,6 +192,7 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     /* box-test candidates [c0, c1) of the staged chunk into a 32-bit mask */
     auto filter = [&](unsigned c0, unsigned c1) {
         unsigned bits = 0;
+#pragma unroll 8
         for (unsigned c = c0; c < c1; ++c) {
             const float *b_lo = chunk_lo[wave][c];
             const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
@@ -298,7 +329,7 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     }
 }
 
-template <class T, unsigned WORDS, class Call>
+template <class T, unsigned WORDS, unsigned CTX_WORDS, class Call>
 __device__ __forceinline__ void tay_act_agents(const TayActArgs &args, Call call) {
     static_assert(sizeof(typename T::agent) <= WORDS * 4u, "agent record larger than the group's agent size");
     const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,7 +343,8 @@ __device__ __forceinline__ void tay_act_agents(const TayActArgs &args, Call call
 #pragma unroll
     for (unsigned k = 0; k < WORDS; ++k)
         orig[k] = A.w[k] = args.cols[k * args.cap + i];
-    call(&A.agent, (typename T::context *)args.context);
+    tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(args.context);
+    call(&A.agent, ctx_holder.get());
 #pragma unroll
     for (unsigned k = 0; k < WORDS; ++k)
         if (A.w[k] != orig[k])
