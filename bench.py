@@ -38,6 +38,14 @@ F_PAIR = 19
 # three accumulators (x, y, z) plus both counts (17 words) and writes the
 # accumulators and counts (11 words); the seen side reads p and v (6 words).
 B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
+# HBM bytes per see launch on this workload, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by what the same
+# counter reads for tay_gpu_copy_probe's 1 GiB copy: FETCH x 2.000, WRITE x
+# 1.000 (profiles/see_flock1M_pmc.md).  A counter run cannot be made from
+# inside this process, so the figure is carried here and re-measured whenever
+# the see kernel changes.
+SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0,
+               "bytes": int((45607.1 * 2.000 + 43211.3 * 1.000) * 1024)}
 
 
 def cpu_model():
@@ -163,7 +171,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": (SEE_TRAFFIC["bytes"] if (args.agents, args.cell_size) ==
+                        (SEE_TRAFFIC["agents"], SEE_TRAFFIC["cell_size"]) else None),
             "kernel": "see (tay_pass_0, flock_see)",
             "note": "FP32 vector-ALU bound, no matrix instructions: the peak is MI355X's FP32 157.3 TFLOP/s "
                     "(FMA counted as 2, packed dual-issue); F = P_see x %d FP32 ops" % F_PAIR,

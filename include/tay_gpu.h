@@ -141,6 +141,13 @@ int tay_gpu_program_set_context(TayGpuProgram *program, unsigned pass_index, con
  * (tay/ocl.c:208-236). */
 int tay_gpu_program_run_pass(TayGpuProgram *program, unsigned pass_index);
 
+/* Seers per lane group when a see tile starts: 64 (the default), 32, 16 or
+ * 8.  A wavefront always owns 64 seers; smaller groups walk tighter cell
+ * boxes one after another, which pays where agents are sparse.  Results do
+ * not depend on it.  No reference counterpart (a tuning knob of this
+ * library). */
+int tay_gpu_program_set_tile_width(TayGpuProgram *program, unsigned width);
+
 /* Pairs that passed the box test (agent_see calls) since the last reset;
  * needs count_pairs.  The reference's narrow-phase telemetry counter
  * (tay/space.c:57-60). */
@@ -151,6 +158,82 @@ int tay_gpu_program_reset_pair_counts(TayGpuProgram *program);
  * measured with HIP events on the library stream when timing is enabled. */
 int tay_gpu_program_enable_timing(TayGpuProgram *program, int enable);
 int tay_gpu_program_pass_time(TayGpuProgram *program, unsigned pass_index, double *ms, unsigned *launches);
+
+/* ------------------------------------------------------ batched ensembles
+ *
+ * No reference counterpart: the reference runs one state at a time
+ * (README.md:13; its process-global statics, tay/space.c:266,
+ * tay/ocl.c:103).  A batch holds many independent instances of one
+ * single-group model (same agent type and passes; own agent count, grid,
+ * radii and contexts) in one set of device columns and advances all of them
+ * with one launch per pass.  Every instance computes exactly what a state of
+ * its own computes with TAY_GPU_Z_GRID and the same cell sizes: the same
+ * sort, the same pairs in the same order, bit for bit.
+ */
+
+typedef struct TayGpuBatch TayGpuBatch;
+
+typedef struct TayGpuBatchPass {
+    int type;                   /* TayGpuPassType */
+    const char *func_name;
+    int self_see;               /* see passes: pairs (i, i) included when 1 */
+    unsigned context_size;      /* bytes of one instance's context */
+} TayGpuBatchPass;
+
+/* counts[b] agents of agent_size bytes per instance; cell_table_bytes[b]
+ * bounds instance b's cell table (8 bytes per cell) as shared_bytes does for
+ * a space.  Compiles the model sources (hipRTC) with one batched kernel per
+ * pass.  Part sizes default to 1 and radii to 0 until set. */
+TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned sources_count,
+                                  const TayGpuBatchPass *passes, unsigned passes_count, unsigned agent_size,
+                                  int dims, unsigned instances, const unsigned *counts,
+                                  const unsigned long long *cell_table_bytes, int count_pairs, char *log,
+                                  unsigned log_size);
+void tay_gpu_batch_destroy(TayGpuBatch *batch);
+
+/* Composes and compiles a batch's program (with pair counting) without a
+ * GPU. */
+int tay_gpu_batch_compile_check(const char *const *sources, unsigned sources_count, const TayGpuBatchPass *passes,
+                                unsigned passes_count, unsigned agent_size, int dims, char *log, unsigned log_size,
+                                unsigned long long *code_size);
+
+/* instances x 4 floats: suggested cell sizes (min_part_sizes) per instance */
+int tay_gpu_batch_set_part_sizes(TayGpuBatch *batch, const float *part_sizes);
+/* instances x 4 floats: the see pass's radii per instance */
+int tay_gpu_batch_set_radii(TayGpuBatch *batch, unsigned pass_index, const float *radii);
+/* instances x context_size bytes, packed: the pass's context per instance */
+int tay_gpu_batch_set_contexts(TayGpuBatch *batch, unsigned pass_index, const void *contexts);
+/* seers per lane group at the start of a see tile: 64, 32, 16 or 8 */
+int tay_gpu_batch_set_tile_width(TayGpuBatch *batch, unsigned width);
+
+/* All instances' agents as AoS records, instance after instance (sum of
+ * counts records). */
+int tay_gpu_batch_push(TayGpuBatch *batch, const void *agents);
+int tay_gpu_batch_fetch(TayGpuBatch *batch, void *agents);
+
+/* Runs `steps` steps (sort every instance, then the passes in order).
+ * use_graph captures one step into a HIP graph (once, and again after a
+ * change of tile width) and replays it; otherwise kernels are launched one
+ * by one, and pass timing, when enabled, records each launch.  No host
+ * synchronisation; *device_ms (optional) receives the device time of the
+ * whole run, which synchronises. */
+int tay_gpu_batch_run(TayGpuBatch *batch, unsigned steps, int use_graph, double *device_ms);
+
+int tay_gpu_batch_enable_timing(TayGpuBatch *batch, int enable);
+int tay_gpu_batch_pass_time(TayGpuBatch *batch, unsigned pass_index, double *ms, unsigned *launches);
+int tay_gpu_batch_pair_count(TayGpuBatch *batch, unsigned pass_index, unsigned long long *pairs);
+int tay_gpu_batch_reset_pair_counts(TayGpuBatch *batch);
+
+/* Autotuner: for every cell factor f (cell size = f x the instance's radius
+ * of the first see pass, per axis) and tile width, restores the agents it
+ * started from, runs `steps` un-captured steps and times them on the device.
+ * table_ms (optional, factors_count x widths_count) receives ms per step.
+ * Applies the fastest pair, reports it and leaves the agents as they were.
+ * The cell size changes the grid, and so the order of each seer's pairs;
+ * the tile width changes neither. */
+int tay_gpu_batch_autotune(TayGpuBatch *batch, const float *cell_factors, unsigned factors_count,
+                           const unsigned *widths, unsigned widths_count, unsigned steps, double *table_ms,
+                           float *best_factor, unsigned *best_width);
 
 #ifdef __cplusplus
 }

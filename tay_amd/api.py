@@ -60,6 +60,10 @@ class TayGpuPassShape(C.Structure):
                 ("seen_kind", C.c_int), ("dims", C.c_int), ("skip_self", C.c_int), ("context_size", C.c_uint)]
 
 
+class TayGpuBatchPass(C.Structure):
+    _fields_ = [("type", C.c_int), ("func_name", C.c_char_p), ("self_see", C.c_int), ("context_size", C.c_uint)]
+
+
 _vp = C.c_void_p
 _u = C.c_uint
 _i = C.c_int
@@ -124,6 +128,25 @@ _GPU = {
     "tay_gpu_program_reset_pair_counts": (_i, [_vp]),
     "tay_gpu_program_enable_timing": (_i, [_vp, _i]),
     "tay_gpu_program_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+    "tay_gpu_program_set_tile_width": (_i, [_vp, _u]),
+    "tay_gpu_batch_create": (_vp, [C.POINTER(C.c_char_p), _u, C.POINTER(TayGpuBatchPass), _u, _u, _i, _u,
+                                   C.POINTER(C.c_uint), C.POINTER(C.c_ulonglong), _i, C.c_char_p, _u]),
+    "tay_gpu_batch_destroy": (None, [_vp]),
+    "tay_gpu_batch_compile_check": (_i, [C.POINTER(C.c_char_p), _u, C.POINTER(TayGpuBatchPass), _u, _u, _i,
+                                         C.c_char_p, _u, C.POINTER(C.c_ulonglong)]),
+    "tay_gpu_batch_set_part_sizes": (_i, [_vp, C.POINTER(C.c_float)]),
+    "tay_gpu_batch_set_radii": (_i, [_vp, _u, C.POINTER(C.c_float)]),
+    "tay_gpu_batch_set_contexts": (_i, [_vp, _u, _vp]),
+    "tay_gpu_batch_set_tile_width": (_i, [_vp, _u]),
+    "tay_gpu_batch_push": (_i, [_vp, _vp]),
+    "tay_gpu_batch_fetch": (_i, [_vp, _vp]),
+    "tay_gpu_batch_run": (_i, [_vp, _u, _i, C.POINTER(C.c_double)]),
+    "tay_gpu_batch_enable_timing": (_i, [_vp, _i]),
+    "tay_gpu_batch_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
+    "tay_gpu_batch_pair_count": (_i, [_vp, _u, C.POINTER(C.c_ulonglong)]),
+    "tay_gpu_batch_reset_pair_counts": (_i, [_vp]),
+    "tay_gpu_batch_autotune": (_i, [_vp, C.POINTER(C.c_float), _u, C.POINTER(C.c_uint), _u, _u,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint)]),
 }
 
 # test-only inspection calls of the oracle (oracle/oracle.h)

@@ -40,6 +40,7 @@ struct TaySeeArgs {
     float radii[4];
     unsigned long long *pair_count;     /* pairs passing the box test, when counting */
     unsigned tiles;                     /* number of 64-seer tiles */
+    unsigned width;                     /* seers per lane group at the start of a tile: 64, 32, 16 or 8 */
 };
 
 struct TayActArgs {
@@ -47,6 +48,49 @@ struct TayActArgs {
     unsigned cap;
     unsigned count;
     const void *context;
+};
+
+/* One instance of a batched ensemble.  Its agents occupy slots
+ * [off, off + count) of the shared columns; off is a multiple of 64, so
+ * every 64-slot tile belongs to one instance, and the slots up to the next
+ * multiple of 64 are padding. */
+struct TayBatchInstance {
+    unsigned off;
+    unsigned count;
+    unsigned rec;                       /* first record of the instance in host (AoS) order */
+    unsigned pad;
+    unsigned long long ranges_off;      /* first entry of its cell table */
+    unsigned long long max_cells;       /* entries of its cell table */
+};
+
+/* Arguments of a generated batched see kernel: one persistent launch for
+ * every instance.  Wavefronts take tiles from a work queue in the order given
+ * by `order` (heaviest first). */
+struct TayBatchSeeArgs {
+    unsigned *cols;
+    unsigned cap;
+    unsigned tiles;
+    const unsigned *tile_inst;          /* instance of each tile */
+    const unsigned *order;              /* tiles in processing order */
+    unsigned *queue;                    /* work-queue head, zero at launch */
+    const struct TayBatchInstance *inst;
+    const struct TayGridGeom *geom;     /* one per instance */
+    const struct TayCellRange *ranges;  /* all cell tables */
+    const float *radii;                 /* 4 per instance */
+    const unsigned *context;            /* ctx_words per instance */
+    unsigned ctx_words;
+    unsigned width;
+    unsigned long long *pair_count;
+};
+
+struct TayBatchActArgs {
+    unsigned *cols;
+    unsigned cap;
+    unsigned slots;
+    const unsigned *tile_inst;
+    const struct TayBatchInstance *inst;
+    const unsigned *context;
+    unsigned ctx_words;
 };
 
 #endif

@@ -31,6 +31,18 @@ void device_free(void *ptr);
 constexpr unsigned kSeeBlock = 256;     /* 4 wavefronts, one 64-seer tile each */
 constexpr unsigned kActBlock = 256;
 
+/* hipRTC compile of a composed program for gfx950 (cached by text); on a
+ * compile error the log goes to log. */
+bool compile_program(const std::string &text, std::vector<char> &code, char *log, unsigned log_size);
+
+/* Program text of a batched ensemble: model sources and one
+ * tay_batch_pass_<i> kernel per pass. */
+std::string compose_batch(const char *const *sources, unsigned sources_count, const TayGpuBatchPass *passes,
+                          unsigned passes_count, unsigned agent_size, int dims, int count_pairs);
+
+/* True for a C identifier. */
+bool valid_func_name(const char *name);
+
 /* Text of device_abi.h and jit_device.h, embedded by the build. */
 extern const char *const device_abi_text;
 extern const char *const jit_device_text;
@@ -89,6 +101,7 @@ struct TayGpuProgram {
     std::vector<Pass> passes;
     int count_pairs;
     int timing;
+    unsigned width;             /* seers per lane group at the start of a see tile */
 };
 
 #endif

@@ -127,8 +127,9 @@ TayGpuPassShape shape_of(const TayGpuPassDesc &d) {
     return sh;
 }
 
-std::string compose(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
-                    unsigned shapes_count, int count_pairs) {
+/* Program text before the generated kernels: ABI structures, the model
+ * dialect, the model sources and the device templates. */
+std::string compose_head(const char *const *sources, unsigned sources_count) {
     std::string text;
     text += device_abi_text;
     text += kPrelude;
@@ -141,14 +142,24 @@ std::string compose(const char *const *sources, unsigned sources_count, const Ta
     text += "#line 1 \"jit_device.h\"\n";
     text += jit_device_text;
     text += "\n#line 1 \"tay_passes\"\n";
-    /* tuning: TAY_SEE_WAVES_PER_EU=<n> asks the compiler to fit n wavefronts
-     * per SIMD into the see kernels' register budget */
-    std::string occupancy;
+    return text;
+}
+
+/* tuning: TAY_SEE_WAVES_PER_EU=<n> asks the compiler to fit n wavefronts per
+ * SIMD into the see kernels' register budget */
+std::string see_occupancy() {
     if (const char *w = std::getenv("TAY_SEE_WAVES_PER_EU")) {
         const int n = std::atoi(w);
         if (n >= 1 && n <= 8)
-            occupancy = " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
+            return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
     }
+    return std::string();
+}
+
+std::string compose(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
+                    unsigned shapes_count, int count_pairs) {
+    std::string text = compose_head(sources, sources_count);
+    const std::string occupancy = see_occupancy();
     for (unsigned i = 0; i < shapes_count; ++i) {
         const TayGpuPassShape &p = shapes[i];
         if (p.type == TAY_GPU_PASS_SEE) {
@@ -176,7 +187,9 @@ std::string compose(const char *const *sources, unsigned sources_count, const Ta
     return text;
 }
 
-bool compile(const std::string &text, std::vector<char> &code, char *log, unsigned log_size) {
+}  // namespace
+
+bool tay::compile_program(const std::string &text, std::vector<char> &code, char *log, unsigned log_size) {
     {
         std::lock_guard<std::mutex> lock(g_cache_mutex);
         auto it = g_code_cache.find(text);
@@ -231,7 +244,37 @@ bool compile(const std::string &text, std::vector<char> &code, char *log, unsign
     return true;
 }
 
-}  // namespace
+std::string tay::compose_batch(const char *const *sources, unsigned sources_count, const TayGpuBatchPass *passes,
+                               unsigned passes_count, unsigned agent_size, int dims, int count_pairs) {
+    std::string text = compose_head(sources, sources_count);
+    const std::string occupancy = see_occupancy();
+    for (unsigned i = 0; i < passes_count; ++i) {
+        const TayGpuBatchPass &p = passes[i];
+        if (p.type == TAY_GPU_PASS_SEE) {
+            append(text, "extern \"C\" __global__ __launch_bounds__(%u)%s", kSeeBlock, occupancy.c_str());
+            append(text,
+                   " void tay_batch_pass_%u(TayBatchSeeArgs args) {\n"
+                   "    typedef tay_see_traits<decltype(%s)> T;\n"
+                   "    tay_see_batch<T, %uu, %s, %d, %s, %uu>(args,\n"
+                   "        [](T::seer *a, T::seen *b, T::context *c) { %s(a, b, c); });\n"
+                   "}\n",
+                   i, p.func_name, agent_size / 4u, p.self_see ? "false" : "true", dims,
+                   count_pairs ? "true" : "false", (p.context_size + 3u) / 4u, p.func_name);
+        } else {
+            append(text,
+                   "extern \"C\" __global__ __launch_bounds__(%u) void tay_batch_pass_%u(TayBatchActArgs args) {\n"
+                   "    typedef tay_act_traits<decltype(%s)> T;\n"
+                   "    tay_act_batch<T, %uu, %uu>(args, [](T::agent *a, T::context *c) { %s(a, c); });\n"
+                   "}\n",
+                   kActBlock, i, p.func_name, agent_size / 4u, (p.context_size + 3u) / 4u, p.func_name);
+        }
+    }
+    return text;
+}
+
+bool tay::valid_func_name(const char *name) {
+    return valid_identifier(name);
+}
 
 TAY_API int tay_gpu_compile_check(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
                                   unsigned shapes_count, char *log, unsigned log_size, unsigned long long *code_size) {
@@ -248,7 +291,7 @@ TAY_API int tay_gpu_compile_check(const char *const *sources, unsigned sources_c
         }
     }
     std::vector<char> code;
-    if (!compile(compose(sources, sources_count, shapes, shapes_count, 0), code, log, log_size))
+    if (!compile_program(compose(sources, sources_count, shapes, shapes_count, 0), code, log, log_size))
         return 0;
     if (code_size)
         *code_size = code.size();
@@ -304,13 +347,14 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
     for (unsigned i = 0; i < passes_count; ++i)
         shapes[i] = shape_of(passes[i]);
     std::vector<char> code;
-    if (!compile(compose(sources, sources_count, shapes.data(), passes_count, count_pairs), code, log, log_size))
+    if (!compile_program(compose(sources, sources_count, shapes.data(), passes_count, count_pairs), code, log, log_size))
         return nullptr;
 
     TayGpuProgram *p = new TayGpuProgram();
     p->module = nullptr;
     p->count_pairs = count_pairs;
     p->timing = 0;
+    p->width = 64;
     if (!hip_ok(hipModuleLoadData(&p->module, code.data()), "hipModuleLoadData")) {
         tay_gpu_program_destroy(p);
         return nullptr;
@@ -423,6 +467,7 @@ TAY_API int tay_gpu_program_run_pass(TayGpuProgram *p, unsigned pass_index) {
             see.radii[k] = d.radii[k];
         see.pair_count = pass.pair_count;
         see.tiles = (a->count + 63u) / 64u;
+        see.width = p->width;
         grid = (see.tiles + (kSeeBlock / 64u) - 1u) / (kSeeBlock / 64u);
         params[0] = &see;
     } else {
@@ -454,6 +499,15 @@ TAY_API int tay_gpu_program_run_pass(TayGpuProgram *p, unsigned pass_index) {
         pass.ev_pending.emplace_back(ev_beg, ev_end);
         ++pass.launches;
     }
+    return 1;
+}
+
+TAY_API int tay_gpu_program_set_tile_width(TayGpuProgram *p, unsigned width) {
+    if (width != 64 && width != 32 && width != 16 && width != 8) {
+        set_error("tile width %u: must be 64, 32, 16 or 8", width);
+        return 0;
+    }
+    p->width = width;
     return 1;
 }
 

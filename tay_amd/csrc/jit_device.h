@@ -16,6 +16,13 @@
  * (tay/cpu_grid.c:222-261) and accumulates exactly the same float sums.
  * Every lane applies the reference's per-axis box test (tay/space.c:62-104)
  * to its own seer.
+ *
+ * Batched kernels (tay_see_batch, tay_act_batch) run the same tile body for
+ * many independent instances sharing one set of columns: a persistent see
+ * grid takes tiles of all instances from a work queue, each tile with its
+ * instance's grid, radii and context.  A seer's pairs, and their order, do
+ * not depend on which tiles run together, so every instance computes exactly
+ * what it computes alone.
  */
 
 template <typename F> struct tay_see_traits;
@@ -134,6 +141,22 @@ __device__ __forceinline__ void tay_wave_sync(void) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* One tile's view of the storage it reads and writes: the plain kernel has
+ * one view for all tiles, a batched kernel one per instance. */
+struct TaySeeView {
+    const unsigned *seer_cols;
+    unsigned *seer_out;
+    unsigned seer_cap;
+    unsigned seer_end;                  /* one past the last seer slot of the tile's group */
+    const unsigned *seen_cols;
+    unsigned seen_cap;
+    unsigned seen_beg, seen_end;        /* the seen slots of a Simple seen space */
+    const TayCellRange *ranges;
+    const TayGridGeom *geom;
+    const void *context;
+    float radii[4];
+};
+
 /* The seen candidates of a tile are consumed in chunks of 64.  Filter: each
  * chunk's boxes are staged in LDS and every lane box-tests all of them
  * (broadcast reads), setting bit c of its 64-bit mask for each passing
@@ -141,10 +164,14 @@ __device__ __forceinline__ void tay_wave_sync(void) {
  * lane taking its lowest remaining bit, so the model body runs with nearly
  * all lanes busy instead of once per candidate under a sparse lane mask.
  * Bits are taken in ascending order and chunks follow the candidate order,
- * so every seer still meets its pairs in the reference's sweep order. */
+ * so every seer still meets its pairs in the reference's sweep order.
+ *
+ * i0 is the tile's first slot; `width` (a power of two up to 64) is the
+ * size of the lane groups a tile starts with.  Returns the pairs this lane
+ * passed to the model when COUNT is set. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
           bool COUNT, unsigned CTX_WORDS, class Call>
-__device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
+__device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned width0, Call call) {
     static_assert(sizeof(typename T::seer) <= SEER_WORDS * 4u, "seer record larger than the group's agent size");
     static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
     constexpr bool SEER_BOX = NARROW == TAY_NARROW_NP || NARROW == TAY_NARROW_NN;
@@ -157,12 +184,8 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
 
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
-    const unsigned tile = tay_xcd_block() * WAVES + wave;
-    if (tile >= args.tiles)
-        return;
-    const unsigned i0 = tile * 64u;
     const unsigned i = i0 + lane;
-    const bool active = i < args.seer_count;
+    const bool active = i < v.seer_end;
     const unsigned ia = active ? i : i0;
 
     union {
@@ -172,7 +195,7 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     unsigned orig[SEER_WORDS];
 #pragma unroll
     for (unsigned k = 0; k < SEER_WORDS; ++k)
-        orig[k] = A.w[k] = args.seer_cols[k * args.seer_cap + ia];
+        orig[k] = A.w[k] = v.seer_cols[k * v.seer_cap + ia];
 
     /* position words: 2.. (min corner), 6.. (max corner of a box agent) */
     float a_lo[3] = {0.0f, 0.0f, 0.0f}, a_hi[3] = {0.0f, 0.0f, 0.0f}, r[3] = {0.0f, 0.0f, 0.0f};
@@ -180,9 +203,9 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     for (int k = 0; k < DIMS; ++k) {
         a_lo[k] = __uint_as_float(A.w[2 + k]);
         a_hi[k] = SEER_BOX ? __uint_as_float(A.w[6 + k]) : a_lo[k];
-        r[k] = args.radii[k];
+        r[k] = v.radii[k];
     }
-    tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(args.context);
+    tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(v.context);
     typename T::context *ctx = ctx_holder.get();
     unsigned pairs = 0;
     bool member = active;   /* lane belongs to the lane group being processed */
@@ -208,10 +231,10 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
         if (lane < fill) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                chunk_lo[wave][lane][k] = k < DIMS ? __uint_as_float(args.seen_cols[(2u + k) * args.seen_cap + cand]) : 0.0f;
+                chunk_lo[wave][lane][k] = k < DIMS ? __uint_as_float(v.seen_cols[(2u + k) * v.seen_cap + cand]) : 0.0f;
                 if (SEEN_BOX)
                     chunk_hi[wave][lane][k] =
-                        k < DIMS ? __uint_as_float(args.seen_cols[(6u + k) * args.seen_cap + cand]) : 0.0f;
+                        k < DIMS ? __uint_as_float(v.seen_cols[(6u + k) * v.seen_cap + cand]) : 0.0f;
             }
             chunk_idx[wave][lane] = cand;
         }
@@ -237,7 +260,7 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
                 } B;
 #pragma unroll
                 for (unsigned k = 0; k < SEEN_WORDS; ++k)
-                    B.w[k] = args.seen_cols[k * args.seen_cap + j];
+                    B.w[k] = v.seen_cols[k * v.seen_cap + j];
 #pragma unroll
                 for (int k = 0; k < DIMS; ++k)
                     B.w[2 + k] = __float_as_uint(chunk_lo[wave][c][k]);
@@ -268,15 +291,15 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
          * depth first.  Seers far apart in space then stop scanning each
          * other's neighbourhoods; each seer still sees its cells in x->y->z
          * order. */
-        const TayGridGeom *g = args.geom;
+        const TayGridGeom *g = v.geom;
         unsigned limit = 1;
 #pragma unroll
         for (int k = 0; k < DIMS; ++k)
             limit *= (unsigned)ceilf(2.0f * r[k] / g->cell_size[k]) + 3u;
         const unsigned cx = (unsigned)g->count[0];
         const unsigned cxy = cx * (unsigned)g->count[1];
-        const unsigned n_active = args.seer_count - i0 < 64u ? args.seer_count - i0 : 64u;
-        unsigned beg = 0, width = 64;
+        const unsigned n_active = v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u;
+        unsigned beg = 0, width = width0;
         while (beg < n_active) {
             member = active && lane - beg < width;
             int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
@@ -301,17 +324,17 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
             for (int x = lo[0]; x <= hi[0]; ++x)
                 for (int y = lo[1]; y <= hi[1]; ++y)
                     for (int z = lo[2]; z <= hi[2]; ++z) {
-                        const TayCellRange cr = args.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                        const TayCellRange cr = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
                         add_range(cr.beg, cr.end);
                     }
             if (fill)
                 process();
             beg += width;
-            while (width < 64u && (beg & (2u * width - 1u)) == 0u)
+            while (width < width0 && (beg & (2u * width - 1u)) == 0u)
                 width <<= 1;
         }
     } else {
-        add_range(0u, args.seen_count);
+        add_range(v.seen_beg, v.seen_end);
         if (fill)
             process();
     }
@@ -320,13 +343,82 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
 #pragma unroll
         for (unsigned k = 0; k < SEER_WORDS; ++k)
             if (A.w[k] != orig[k])
-                args.seer_out[k * args.seer_cap + i] = A.w[k];
+                v.seer_out[k * v.seer_cap + i] = A.w[k];
     }
-    if (COUNT) {
-        unsigned total = tay_wave_sum(pairs);
-        if (lane == 0u && total)
-            atomicAdd(args.pair_count, (unsigned long long)total);
+    return pairs;
+}
+
+__device__ __forceinline__ void tay_add_pairs(unsigned long long *counter, unsigned pairs) {
+    const unsigned total = tay_wave_sum(pairs);
+    if ((threadIdx.x & 63u) == 0u && total)
+        atomicAdd(counter, (unsigned long long)total);
+}
+
+/* Plain see kernel: wavefront k of the grid owns tile k. */
+template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
+          bool COUNT, unsigned CTX_WORDS, class Call>
+__device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
+    constexpr unsigned WAVES = TAY_SEE_BLOCK / 64u;
+    const unsigned tile = tay_xcd_block() * WAVES + (threadIdx.x >> 6);
+    if (tile >= args.tiles)
+        return;
+    TaySeeView v;
+    v.seer_cols = args.seer_cols;
+    v.seer_out = args.seer_out;
+    v.seer_cap = args.seer_cap;
+    v.seer_end = args.seer_count;
+    v.seen_cols = args.seen_cols;
+    v.seen_cap = args.seen_cap;
+    v.seen_beg = 0;
+    v.seen_end = args.seen_count;
+    v.ranges = args.ranges;
+    v.geom = args.geom;
+    v.context = args.context;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        v.radii[k] = args.radii[k];
+    const unsigned pairs = tay_see_tile<T, SEER_WORDS, SEEN_WORDS, NARROW, SKIP_SELF, SEEN_GRID, DIMS, COUNT,
+                                        CTX_WORDS>(v, tile * 64u, args.width, call);
+    if (COUNT)
+        tay_add_pairs(args.pair_count, pairs);
+}
+
+/* Batched see kernel: a persistent grid whose wavefronts take tiles from a
+ * work queue until it is empty; every wavefront reaches the exit test after
+ * each tile, so the grid drains. */
+template <class T, unsigned WORDS, bool SKIP_SELF, int DIMS, bool COUNT, unsigned CTX_WORDS, class Call>
+__device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call call) {
+    unsigned pairs = 0;
+    for (;;) {
+        unsigned k = 0;
+        if ((threadIdx.x & 63u) == 0u)
+            k = atomicAdd(args.queue, 1u);
+        k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+        if (k >= args.tiles)
+            break;
+        const unsigned tile = args.order[k];
+        const unsigned b = args.tile_inst[tile];
+        const TayBatchInstance inst = args.inst[b];
+        TaySeeView v;
+        v.seer_cols = args.cols;
+        v.seer_out = args.cols;
+        v.seer_cap = args.cap;
+        v.seer_end = inst.off + inst.count;
+        v.seen_cols = args.cols;
+        v.seen_cap = args.cap;
+        v.seen_beg = inst.off;
+        v.seen_end = inst.off + inst.count;
+        v.ranges = args.ranges + inst.ranges_off;
+        v.geom = args.geom + b;
+        v.context = args.context + (unsigned long long)b * args.ctx_words;
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+            v.radii[d] = args.radii[4u * b + d];
+        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS>(
+            v, tile * 64u, args.width, call);
     }
+    if (COUNT)
+        tay_add_pairs(args.pair_count, pairs);
 }
 
 template <class T, unsigned WORDS, unsigned CTX_WORDS, class Call>
@@ -344,6 +436,34 @@ __device__ __forceinline__ void tay_act_agents(const TayActArgs &args, Call call
     for (unsigned k = 0; k < WORDS; ++k)
         orig[k] = A.w[k] = args.cols[k * args.cap + i];
     tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(args.context);
+    call(&A.agent, ctx_holder.get());
+#pragma unroll
+    for (unsigned k = 0; k < WORDS; ++k)
+        if (A.w[k] != orig[k])
+            args.cols[k * args.cap + i] = A.w[k];
+}
+
+/* Batched act kernel: one thread per slot of every instance; padding slots
+ * do nothing. */
+template <class T, unsigned WORDS, unsigned CTX_WORDS, class Call>
+__device__ __forceinline__ void tay_act_batch(const TayBatchActArgs &args, Call call) {
+    static_assert(sizeof(typename T::agent) <= WORDS * 4u, "agent record larger than the group's agent size");
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= args.slots)
+        return;
+    const unsigned b = args.tile_inst[i >> 6];
+    const TayBatchInstance inst = args.inst[b];
+    if (i - inst.off >= inst.count)
+        return;
+    union {
+        typename T::agent agent;
+        unsigned w[WORDS];
+    } A;
+    unsigned orig[WORDS];
+#pragma unroll
+    for (unsigned k = 0; k < WORDS; ++k)
+        orig[k] = A.w[k] = args.cols[k * args.cap + i];
+    tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(args.context + (unsigned long long)b * args.ctx_words);
     call(&A.agent, ctx_holder.get());
 #pragma unroll
     for (unsigned k = 0; k < WORDS; ++k)

@@ -13,6 +13,7 @@ import numpy as np
 
 from . import agents as A
 from .api import float4
+from .api import TayGpuBatchPass as api_batch_pass
 
 
 def rng(seed):
@@ -251,6 +252,121 @@ class Simulation:
             self.lib.tay_simulation_end(self.state)
             self.lib.tay_destroy_state(self.state)
             self.state = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def ensemble(instances=256, seed=7, counts=(1000, 64000), boxes=(200.0, 1500.0), radii=(10.0, 40.0),
+             weights=(0.5, 2.0), dt=0.1, max_speed=2.0):
+    """Batched-ensemble workload: per instance an agent count log-uniform in
+    `counts`, a box side uniform in `boxes`, a see radius uniform in `radii`
+    and three flocking weights uniform in `weights`; agents uniform in the
+    instance's box with unit-random velocities.  Returns a list of
+    (records, model, box side) per instance."""
+    g = rng(seed)
+    n = np.exp(g.uniform(np.log(counts[0]), np.log(counts[1]), instances)).astype(np.int64)
+    side = g.uniform(boxes[0], boxes[1], instances)
+    r = g.uniform(radii[0], radii[1], instances).astype(np.float32)
+    w = g.uniform(weights[0], weights[1], (instances, 3)).astype(np.float32)
+    out = []
+    for b in range(instances):
+        gb = rng(seed * 100003 + b)
+        rec = np.zeros(int(n[b]), A.FLOCK_AGENT)
+        rec["p"] = _xyz0(uniform_points(int(n[b]), 0.0, float(side[b]), gb))
+        rec["v"] = _xyz0(unit_vectors(int(n[b]), gb))
+        rec["result_index"] = np.arange(int(n[b]))
+        out.append((rec, flock_model(radius=float(r[b]), weights=tuple(float(x) for x in w[b]), dt=dt,
+                                     max_speed=max_speed), float(side[b])))
+    return out
+
+
+def cell_table_mb(side, part, headroom=2.0):
+    """Whole megabytes of cell table for a cubic box of `side` at cell size
+    `part`, with `headroom` per axis for the box to grow."""
+    cells = (np.ceil(side * headroom / part) + 1.0) ** 3
+    return int(max(1, np.ceil(cells * 8 / (1 << 20))))
+
+
+class Batch:
+    """A batched ensemble on the GPU library: every instance runs `model`'s
+    passes (same functions, own contexts and radii) over its own records.
+
+    part[b] is instance b's cell size and table_mb[b] its cell table in whole
+    megabytes, so a state built with tay_configure_space(..., part[b],
+    table_mb[b]) computes the same grid."""
+
+    def __init__(self, lib, instances, part, table_mb, count_pairs=False, sources=None):
+        self.lib = lib
+        self.models = [m for _, m, _ in instances]
+        self.records = [r for r, _, _ in instances]
+        m0 = self.models[0]
+        self.dtype = m0.dtype
+        texts = [t.encode() for t in (A.source_texts() if sources is None else sources)]
+        self._src = (C.c_char_p * len(texts))(*texts)
+        self.passes = (api_batch_pass * len(m0.passes))(
+            *[api_batch_pass(0 if ps.kind == "see" else 1, ps.func.encode(), ps.self_see,
+                             ps.context.itemsize if ps.context is not None else 0) for ps in m0.passes])
+        self.counts = np.array([len(r) for r in self.records], np.uint32)
+        B = len(self.records)
+        tables = (C.c_ulonglong * B)(*[int(mb) << 20 for mb in table_mb])
+        log = C.create_string_buffer(1 << 16)
+        self.batch = lib.tay_gpu_batch_create(self._src, len(texts), self.passes, len(m0.passes),
+                                              m0.dtype.itemsize, m0.dims, B,
+                                              self.counts.ctypes.data_as(C.POINTER(C.c_uint)), tables,
+                                              1 if count_pairs else 0, log, len(log))
+        if not self.batch:
+            raise RuntimeError("tay_gpu_batch_create: %s\n%s" % (lib.tay_gpu_last_error().decode(), log.value.decode()))
+        self.set_part(part)
+        for i, ps in enumerate(m0.passes):
+            if ps.kind == "see":
+                radii = np.zeros((B, 4), np.float32)
+                radii[:, :3] = np.array([m.passes[i].radius for m in self.models], np.float32)[:, None]
+                radii[:, 3] = radii[:, 0]
+                self._ok(lib.tay_gpu_batch_set_radii(self.batch, i, radii.ctypes.data_as(C.POINTER(C.c_float))))
+            if ps.context is not None:
+                ctx = np.concatenate([m.passes[i].context for m in self.models])
+                self._ok(lib.tay_gpu_batch_set_contexts(self.batch, i, ctx.ctypes.data))
+        allrec = np.concatenate(self.records)
+        self._ok(lib.tay_gpu_batch_push(self.batch, allrec.ctypes.data))
+
+    def _ok(self, r):
+        if not r:
+            raise RuntimeError(self.lib.tay_gpu_last_error().decode())
+        return r
+
+    def set_part(self, part):
+        parts = np.zeros((len(self.records), 4), np.float32)
+        parts[:] = np.asarray(part, np.float32)[:, None]
+        self._ok(self.lib.tay_gpu_batch_set_part_sizes(self.batch, parts.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def run(self, steps, graph=True):
+        ms = C.c_double()
+        self._ok(self.lib.tay_gpu_batch_run(self.batch, steps, 1 if graph else 0, C.byref(ms)))
+        return ms.value
+
+    def agents(self):
+        out = np.zeros(int(self.counts.sum()), self.dtype)
+        self._ok(self.lib.tay_gpu_batch_fetch(self.batch, out.ctypes.data))
+        return np.split(out, np.cumsum(self.counts)[:-1].astype(np.int64))
+
+    def pair_count(self, pass_index):
+        n = C.c_ulonglong()
+        self._ok(self.lib.tay_gpu_batch_pair_count(self.batch, pass_index, C.byref(n)))
+        return n.value
+
+    def pass_time(self, pass_index):
+        ms, launches = C.c_double(), C.c_uint()
+        self._ok(self.lib.tay_gpu_batch_pass_time(self.batch, pass_index, C.byref(ms), C.byref(launches)))
+        return ms.value, launches.value
+
+    def close(self):
+        if self.batch:
+            self.lib.tay_gpu_batch_destroy(self.batch)
+            self.batch = None
 
     def __enter__(self):
         return self

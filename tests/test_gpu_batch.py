@@ -1,0 +1,128 @@
+"""Batched ensembles on the GPU: every instance of a batch must compute,
+bit for bit, what a state of its own computes with TAY_GPU_Z_GRID and the
+same cell size and cell table, which in turn is the oracle's result.  The
+captured-graph path must match the launch-by-launch path on every replay.
+"""
+import numpy as np
+import pytest
+
+from tay_amd import api
+from tay_amd import workloads as W
+
+from test_gpu_parity import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def small_ensemble(instances=12, seed=3):
+    return W.ensemble(instances, seed=seed, counts=(40, 3000), boxes=(60.0, 400.0), radii=(10.0, 40.0))
+
+
+def parts_and_tables(ens, factor=1.0):
+    part = [factor * m.passes[0].radius for _, m, _ in ens]
+    tables = [W.cell_table_mb(side, p) for (_, _, side), p in zip(ens, part)]
+    return part, tables
+
+
+def per_instance(lib, ens, part, tables, steps, funcs=None, space=api.TAY_GPU_Z_GRID):
+    out, pairs = [], []
+    for (rec, model, _), p, mb in zip(ens, part, tables):
+        with W.Simulation(lib, model, rec, space, funcs=funcs, shared_mb=mb, part=p) as s:
+            if funcs is None:
+                lib.tay_set_pair_counting(s.state, 1)
+            s.run(steps)
+            out.append(s.agents())
+            pairs.append(lib.pair_counts(s.state, 0)[1])
+    return out, pairs
+
+
+def test_batch_matches_per_instance_states(gpu):
+    ens = small_ensemble()
+    part, tables = parts_and_tables(ens)
+    print("case: batch 12 instances, 4 steps, eager", flush=True)
+    with W.Batch(gpu, ens, part, tables, count_pairs=True) as b:
+        b.run(4, graph=False)
+        got = b.agents()
+        bpairs = b.pair_count(0)
+    want, pairs = per_instance(gpu, ens, part, tables, 4)
+    for g, w in zip(got, want):
+        assert_bitwise(g, w)
+    assert bpairs == sum(pairs)
+
+
+def test_batch_matches_oracle(gpu, oracle, models):
+    ens = small_ensemble(5, seed=4)
+    part, tables = parts_and_tables(ens, 1.5)
+    print("case: batch 5 instances, 3 steps, graph, vs oracle", flush=True)
+    with W.Batch(gpu, ens, part, tables) as b:
+        b.run(3, graph=True)
+        got = b.agents()
+    want, _ = per_instance(oracle, ens, part, tables, 3, funcs=models)
+    for g, w in zip(got, want):
+        assert_bitwise(g, w)
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    ens = small_ensemble(9, seed=5)
+    part, tables = parts_and_tables(ens)
+    print("case: batch graph vs eager, 3 runs of 5 steps", flush=True)
+    with W.Batch(gpu, ens, part, tables, count_pairs=True) as g, W.Batch(gpu, ens, part, tables,
+                                                                        count_pairs=True) as e:
+        for run in range(3):
+            g.run(5, graph=True)
+            e.run(5, graph=False)
+            gpu.tay_gpu_synchronize()
+            for x, y in zip(g.agents(), e.agents()):
+                assert_bitwise(x, y)
+        assert g.pair_count(0) == e.pair_count(0)
+
+
+def test_batch_tile_width_does_not_change_results(gpu):
+    ens = small_ensemble(6, seed=6)
+    part, tables = parts_and_tables(ens)
+    print("case: batch tile widths 64/32/8", flush=True)
+    results = []
+    for width in (64, 32, 8):
+        with W.Batch(gpu, ens, part, tables) as b:
+            assert gpu.tay_gpu_batch_set_tile_width(b.batch, width)
+            b.run(3, graph=True)
+            results.append(b.agents())
+    for other in results[1:]:
+        for x, y in zip(results[0], other):
+            assert_bitwise(x, y)
+
+
+def test_batch_autotune_restores_agents(gpu):
+    import ctypes as C
+    ens = small_ensemble(6, seed=8)
+    part, _ = parts_and_tables(ens)
+    tables = [W.cell_table_mb(side, 0.5 * m.passes[0].radius) for _, m, side in ens]
+    print("case: batch autotune 3 factors x 2 widths", flush=True)
+    with W.Batch(gpu, ens, part, tables) as b:
+        before = b.agents()
+        factors = (C.c_float * 3)(1.0, 1.5, 2.0)
+        widths = (C.c_uint * 2)(64, 16)
+        table = (C.c_double * 6)()
+        bf, bw = C.c_float(), C.c_uint()
+        assert gpu.tay_gpu_batch_autotune(b.batch, factors, 3, widths, 2, 2, table, C.byref(bf), C.byref(bw)), \
+            gpu.tay_gpu_last_error().decode()
+        assert bf.value in (1.0, 1.5, 2.0) and bw.value in (64, 16)
+        assert all(t > 0 for t in table)
+        for x, y in zip(before, b.agents()):
+            assert_bitwise(x, y)
+        b.run(2, graph=True)
+        got = b.agents()
+    chosen = [bf.value * m.passes[0].radius for _, m, _ in ens]
+    want, _ = per_instance(gpu, ens, chosen, tables, 2)
+    for x, y in zip(got, want):
+        assert_bitwise(x, y)
+
+
+def test_batch_rejects_bad_arguments(gpu):
+    ens = small_ensemble(2)
+    part, tables = parts_and_tables(ens)
+    with W.Batch(gpu, ens, part, tables) as b:
+        assert not gpu.tay_gpu_batch_set_tile_width(b.batch, 48)
+        assert b"tile width" in gpu.tay_gpu_last_error()
+        assert not gpu.tay_gpu_batch_set_radii(b.batch, 1, None)
+        assert b"not a see pass" in gpu.tay_gpu_last_error()

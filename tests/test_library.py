@@ -68,6 +68,28 @@ def test_models_compile_for_gfx950_without_gpu(tay):
     assert size > 1000
 
 
+def test_batched_programs_compile_without_gpu(tay):
+    P = api.TayGpuBatchPass
+    srcs = [s.encode() for s in A.source_texts()]
+    arr = (C.c_char_p * len(srcs))(*srcs)
+    for dtype, passes in ((A.FLOCK_AGENT, [P(0, b"flock_see", 0, A.FLOCK_SEE_CONTEXT.itemsize),
+                                           P(1, b"flock_act", 0, A.FLOCK_ACT_CONTEXT.itemsize)]),
+                          (A.SPH_PARTICLE, [P(0, b"sph_density", 1, A.SPH_CONTEXT.itemsize),
+                                            P(1, b"sph_pressure", 0, A.SPH_CONTEXT.itemsize),
+                                            P(0, b"sph_forces", 0, A.SPH_CONTEXT.itemsize),
+                                            P(1, b"sph_integrate", 0, A.SPH_CONTEXT.itemsize)])):
+        ps = (P * len(passes))(*passes)
+        log = C.create_string_buffer(1 << 16)
+        size = C.c_ulonglong()
+        ok = tay.tay_gpu_batch_compile_check(arr, len(srcs), ps, len(passes), dtype.itemsize, 3, log, len(log),
+                                             C.byref(size))
+        assert ok, log.value.decode(errors="replace") + tay.tay_gpu_last_error().decode()
+        assert size.value > 1000
+    bad = (P * 1)(P(0, b"not a name", 0, 0))
+    assert not tay.tay_gpu_batch_compile_check(arr, len(srcs), bad, 1, A.FLOCK_AGENT.itemsize, 3, None, 0, None)
+    assert b"identifier" in tay.tay_gpu_last_error()
+
+
 def test_compile_error_reports_the_log(tay):
     broken = A.source_texts() + ["void plain_see_broken(global PlainAgent *a) { a->no_such_field = 1; }"]
     ok, log, _ = compile_check(tay, broken, [_shape(1, "plain_act", A.PLAIN_AGENT)])
