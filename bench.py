@@ -84,6 +84,120 @@ def cpu_baseline(recs, model, threads, steps):
     }
 
 
+def ensemble_cpu_baseline(ens, part, tables, threads):
+    """One step of every instance on the oracle's CpuGrid, states run one
+    after another as the reference allows."""
+    from tay_amd import api
+    from tay_amd import workloads as W
+    oracle = api.TayLib(os.path.join(ROOT, "oracle", "_build", "liboracle.so"))
+    models = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "libmodels.so"))
+    oracle.tay_threads_start(threads, 0)
+    wall = see_s = 0.0
+    pairs = 0
+    for (rec, model, _), p, mb in zip(ens, part, tables):
+        with W.Simulation(oracle, model, rec, api.TAY_CPU_GRID, funcs=models, shared_mb=mb, part=p) as s:
+            t0 = time.perf_counter()
+            s.run(1)
+            wall += time.perf_counter() - t0
+            see_s += oracle.oracle_get_see_seconds(s.state)
+            pairs += oracle.pair_counts(s.state, 0)[1]
+    oracle.tay_threads_stop()
+    return {"value": pairs / see_s if see_s else 0.0, "unit": "pair interactions/s", "cores": threads,
+            "kind": "port", "steps_per_s": 1.0 / wall,
+            "sample": "1 step of each of the %d instances on CpuGrid, one state after another; %.1f s wall"
+                      % (len(ens), wall)}
+
+
+def bench_ensemble(lib, args):
+    """Batched ensemble: every instance in one launch per pass (HIP-graph
+    step), against one state per instance run one after another through the
+    same library (tay.h, TAY_GPU_Z_GRID, same cell sizes, same results)."""
+    import numpy as np
+    from tay_amd import api
+    from tay_amd import workloads as W
+    ens = W.ensemble(args.ensemble_instances, seed=7)
+    radius = [m.passes[0].radius for _, m, _ in ens]
+    factors, widths = (1.0, 1.5, 2.0), (64, 32, 16)
+    # tables sized for the smallest cell factor, shared by every run below
+    tables = [W.cell_table_mb(side, min(factors) * r) for (_, _, side), r in zip(ens, radius)]
+    steps, warm = args.ensemble_steps, 2
+
+    with W.Batch(lib, ens, radius, tables) as b:
+        f = (ctypes.c_float * len(factors))(*factors)
+        w = (ctypes.c_uint * len(widths))(*widths)
+        table = (ctypes.c_double * (len(factors) * len(widths)))()
+        bf, bw = ctypes.c_float(), ctypes.c_uint()
+        if not lib.tay_gpu_batch_autotune(b.batch, f, len(factors), w, len(widths), 3, table, ctypes.byref(bf),
+                                          ctypes.byref(bw)):
+            raise SystemExit("autotune: " + lib.tay_gpu_last_error().decode())
+    part = [bf.value * r for r in radius]
+    width = bw.value
+
+    with W.Batch(lib, ens, part, tables, count_pairs=True) as twin:
+        lib.tay_gpu_batch_set_tile_width(twin.batch, width)
+        twin.run(warm)
+        lib.tay_gpu_batch_reset_pair_counts(twin.batch)
+        twin.run(steps)
+        p_see = twin.pair_count(0)
+
+    with W.Batch(lib, ens, part, tables) as b:
+        lib.tay_gpu_batch_set_tile_width(b.batch, width)
+        b.run(warm, graph=False)
+        lib.tay_gpu_batch_enable_timing(b.batch, 1)
+        b.run(steps, graph=False)
+        see_ms, launches = b.pass_time(0)
+        act_ms, _ = b.pass_time(1)
+
+    with W.Batch(lib, ens, part, tables) as b:
+        lib.tay_gpu_batch_set_tile_width(b.batch, width)
+        b.run(warm, graph=True)
+        lib.tay_gpu_synchronize()
+        t0 = time.perf_counter()
+        dev_ms = b.run(steps, graph=True)
+        wall = time.perf_counter() - t0
+        batched = b.agents()
+
+    inst_wall = inst_see_ms = 0.0
+    identical = True
+    for (rec, model, _), p, mb, got in zip(ens, part, tables, batched):
+        with W.Simulation(lib, model, rec, api.TAY_GPU_Z_GRID, shared_mb=mb, part=p) as s:
+            lib.tay_set_fetch_mode(s.state, api.TAY_FETCH_ON_DEMAND)
+            lib.tay_set_tile_width(s.state, width)
+            s.run(warm)
+            lib.tay_set_pass_timing(s.state, 1)
+            lib.tay_gpu_synchronize()
+            t0 = time.perf_counter()
+            s.run(steps)
+            lib.tay_gpu_synchronize()
+            inst_wall += time.perf_counter() - t0
+            inst_see_ms += lib.pass_time(s.state, 0)[0]
+            lib.tay_fetch_agents(s.state, None)
+            identical = identical and np.array_equal(s.agents().view(np.uint8), got.view(np.uint8))
+
+    out = {
+        "workload": "flocking ensemble, %d instances" % len(ens),
+        "instances": len(ens), "agents": int(sum(len(r) for r, _, _ in ens)), "steps": steps, "warmup": warm,
+        "p_see_per_step": p_see / steps,
+        "batched": {"pairs_per_s": p_see / (see_ms / 1e3), "steps_per_s": steps / wall,
+                    "ms_per_step": wall * 1e3 / steps, "device_ms_per_step": dev_ms / steps,
+                    "see_ms_per_step": see_ms / launches, "act_ms_per_step": act_ms / launches,
+                    "launches_per_pass_per_step": 1},
+        "per_instance": {"pairs_per_s": p_see / (inst_see_ms / 1e3), "steps_per_s": steps / inst_wall,
+                         "ms_per_step": inst_wall * 1e3 / steps, "see_ms_per_step": inst_see_ms / steps,
+                         "launches_per_pass_per_step": len(ens)},
+        "identical": bool(identical),
+        "autotune": {"cell_factors": list(factors), "tile_widths": list(widths),
+                     "ms_per_step": [[table[i * len(widths) + j] for j in range(len(widths))]
+                                     for i in range(len(factors))],
+                     "chosen": {"cell_factor": bf.value, "tile_width": width}},
+    }
+    out["speedup_pairs_per_s"] = out["batched"]["pairs_per_s"] / out["per_instance"]["pairs_per_s"]
+    out["speedup_steps_per_s"] = out["batched"]["steps_per_s"] / out["per_instance"]["steps_per_s"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = ensemble_cpu_baseline(ens, part, tables, args.cpu_threads)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +208,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-instances", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one plain device-to-device copy of MIB MiB (reads and writes MIB MiB), the "
                          "calibration kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -188,6 +305,8 @@ def main():
                              "GB/s": 2 * (args.copy_probe << 20) / probe_ms.value / 1e6}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(recs, model, args.cpu_threads, args.cpu_steps)
+    if not args.no_ensemble:
+        out["ensemble"] = bench_ensemble(lib, args)
     print(json.dumps(out))
     return 0
 

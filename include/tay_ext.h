@@ -32,6 +32,10 @@ int tay_get_pair_count(TayState *state, unsigned pass_index, unsigned long long 
 void tay_set_pass_timing(TayState *state, int enable);
 int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, unsigned *launches);
 
+/* Seers per lane group when a see tile starts (64, 32, 16 or 8; default
+ * 64), as tay_gpu_program_set_tile_width; results do not depend on it. */
+int tay_set_tile_width(TayState *state, unsigned width);
+
 /* The device space of a group, valid while the simulation runs (NULL before
  * the first tay_run and for groups not on the GPU). */
 TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group);

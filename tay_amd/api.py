@@ -102,6 +102,7 @@ _GPU = {
     "tay_set_pass_timing": (None, [_vp, _i]),
     "tay_get_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_get_gpu_space": (_vp, [_vp, _vp]),
+    "tay_set_tile_width": (_i, [_vp, _u]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),
     "tay_gpu_available": (_i, []),

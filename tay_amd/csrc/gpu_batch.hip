@@ -10,8 +10,9 @@
  * (ranges + ranges_off_b).
  *
  * Sort, per step, for every instance at once:
- *   box       one block per instance: min/max of its positions
- *   geometry  one thread per instance: the single-space geometry
+ *   box       one wavefront per tile: min/max of its positions
+ *   geometry  one wavefront per instance: reduce its tiles' boxes, then the
+ *             single-space geometry
  *   keys      per slot: the instance-local Morton key, padding 0xFFFFFFFF
  *   radix     4 stable passes over the key bytes, then 1 (or 2) stable passes
  *             over the instance index: instance-major, key-minor, and within
@@ -42,63 +43,61 @@ namespace {
 
 constexpr unsigned kCostBuckets = 32;
 
-/* One block per instance: bounding box of its positions. */
-__global__ __launch_bounds__(kBuildBlock) void k_batch_box(const unsigned *__restrict__ cols, unsigned cap,
-                                                           const TayBatchInstance *__restrict__ inst, int dims,
-                                                           float *__restrict__ boxes) {
-    const TayBatchInstance in = inst[blockIdx.x];
-    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
-    float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (unsigned i = in.off + threadIdx.x; i < in.off + in.count; i += kBuildBlock) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (k < dims) {
-                const float p = __uint_as_float(cols[(2u + k) * cap + i]);
-                lo[k] = fminf(lo[k], p);
-                hi[k] = fmaxf(hi[k], p);
-            }
-        }
-    }
-    __shared__ float s[kBuildBlock / 64][6];
-    const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+/* One wavefront per tile: bounding box of the tile's agents. */
+__global__ __launch_bounds__(kBuildBlock) void k_tile_box(const unsigned *__restrict__ cols, unsigned cap,
+                                                          unsigned tiles, const unsigned *__restrict__ tile_inst,
+                                                          const TayBatchInstance *__restrict__ inst, int dims,
+                                                          float *__restrict__ partials) {
+    const unsigned t = blockIdx.x * (kBuildBlock / 64u) + (threadIdx.x >> 6);
+    if (t >= tiles)
+        return;
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned i = t * 64u + lane;
+    const TayBatchInstance in = inst[tile_inst[t]];
+    const bool real = i - in.off < in.count;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float a = wave_min(lo[k]), b = wave_max(hi[k]);
+        const float p = real && k < dims ? __uint_as_float(cols[(2u + k) * cap + i]) : 0.0f;
+        const float lo = wave_min(real ? p : FLT_MAX), hi = wave_max(real ? p : -FLT_MAX);
         if (lane == 0) {
-            s[w][k] = a;
-            s[w][3 + k] = b;
+            partials[t * 6u + k] = lo;
+            partials[t * 6u + 3 + k] = hi;
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        const unsigned k = threadIdx.x;
-        float v = s[0][k];
-        for (unsigned j = 1; j < kBuildBlock / 64; ++j)
-            v = k < 3 ? fminf(v, s[j][k]) : fmaxf(v, s[j][k]);
-        boxes[blockIdx.x * 6u + k] = v;
     }
 }
 
-/* One thread per instance: its grid, as tay_gpu_space_sort derives it. */
-__global__ __launch_bounds__(64) void k_batch_geometry(const float *__restrict__ boxes,
+/* One wavefront per instance: reduce its tiles' boxes, then lane 0 derives
+ * the grid as tay_gpu_space_sort does. */
+__global__ __launch_bounds__(64) void k_batch_geometry(const float *__restrict__ partials,
                                                        const float *__restrict__ parts,
-                                                       const TayBatchInstance *__restrict__ inst, unsigned instances,
-                                                       int dims, TayGridGeom *__restrict__ geom) {
-    const unsigned b = blockIdx.x * 64u + threadIdx.x;
-    if (b >= instances)
-        return;
-    float lo[3], hi[3], part[3];
+                                                       const TayBatchInstance *__restrict__ inst, int dims,
+                                                       TayGridGeom *__restrict__ geom) {
+    const unsigned b = blockIdx.x, lane = threadIdx.x;
+    const TayBatchInstance in = inst[b];
+    const unsigned t0 = in.off / 64u, t1 = (in.off + in.count + 63u) / 64u;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (unsigned t = t0 + lane; t < t1; t += 64u) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], partials[t * 6u + k]);
+            hi[k] = fmaxf(hi[k], partials[t * 6u + 3 + k]);
+        }
+    }
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-        lo[k] = boxes[b * 6u + k];
-        hi[k] = boxes[b * 6u + 3 + k];
+        lo[k] = wave_min(lo[k]);
+        hi[k] = wave_max(hi[k]);
+    }
+    if (lane != 0)
+        return;
+    float part[3];
+    for (int k = 0; k < 3; ++k) {
         part[k] = parts[b * 4u + k];
+        if (in.count == 0)
+            lo[k] = hi[k] = 0.0f;   /* no agents: a one-cell grid that no tile reads */
     }
-    if (inst[b].count == 0) {
-        /* no agents: a one-cell grid that no tile reads */
-        for (int k = 0; k < 3; ++k)
-            lo[k] = hi[k] = 0.0f;
-    }
-    grid_geometry(lo, hi, part, dims, inst[b].max_cells, geom + b);
+    grid_geometry(lo, hi, part, dims, in.max_cells, geom + b);
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_batch_keys(const unsigned *__restrict__ cols, unsigned cap,
@@ -187,7 +186,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_batch_permute(const unsigned *_
         dst[w * cap + i] = src[w * cap + from];
 }
 
-/* Estimated cost of each tile, bucketed by its binary logarithm. */
+/* Estimated cost of each tile, bucketed by its binary logarithm; the block
+ * counts its tiles per bucket in LDS and adds the counts to the histogram. */
 __global__ __launch_bounds__(kBuildBlock) void k_tile_cost(const unsigned *__restrict__ keys, unsigned tiles,
                                                            const unsigned *__restrict__ tile_inst,
                                                            const TayBatchInstance *__restrict__ inst, int dims,
@@ -195,19 +195,26 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_cost(const unsigned *__res
                                                            const TayCellRange *__restrict__ ranges,
                                                            unsigned *__restrict__ tile_bucket,
                                                            unsigned *__restrict__ bucket_hist) {
+    __shared__ unsigned h[kCostBuckets];
+    if (threadIdx.x < kCostBuckets)
+        h[threadIdx.x] = 0u;
+    __syncthreads();
     const unsigned t = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (t >= tiles)
-        return;
-    const unsigned b = tile_inst[t];
-    const TayBatchInstance in = inst[b];
-    const unsigned s0 = t * 64u;
-    const unsigned n = in.off + in.count - s0 < 64u ? in.off + in.count - s0 : 64u;
-    const unsigned mid = s0 + n / 2u;
-    const TayCellRange r = ranges[in.ranges_off + key_to_cell(keys[mid], dims, geom + b)];
-    const unsigned cost = (r.end - r.beg) * n;
-    const unsigned bucket = cost ? 31u - (unsigned)__clz(cost) : 0u;
-    tile_bucket[t] = bucket;
-    atomicAdd(&bucket_hist[bucket], 1u);
+    if (t < tiles) {
+        const unsigned b = tile_inst[t];
+        const TayBatchInstance in = inst[b];
+        const unsigned s0 = t * 64u;
+        const unsigned n = in.off + in.count - s0 < 64u ? in.off + in.count - s0 : 64u;
+        const unsigned mid = s0 + n / 2u;
+        const TayCellRange r = ranges[in.ranges_off + key_to_cell(keys[mid], dims, geom + b)];
+        const unsigned cost = (r.end - r.beg) * n;
+        const unsigned bucket = cost ? 31u - (unsigned)__clz(cost) : 0u;
+        tile_bucket[t] = bucket;
+        atomicAdd(&h[bucket], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kCostBuckets && h[threadIdx.x])
+        atomicAdd(&bucket_hist[threadIdx.x], h[threadIdx.x]);
 }
 
 /* One wavefront: bucket offsets, heaviest bucket first; clears the histogram
@@ -227,13 +234,26 @@ __global__ __launch_bounds__(64) void k_tile_scan(unsigned *__restrict__ bucket_
         queues[lane] = 0u;
 }
 
+/* Places each tile in its bucket's run of the order: ranks inside the block
+ * from LDS counters, one reservation per bucket and block.  The order inside
+ * a bucket is not fixed, which no result depends on. */
 __global__ __launch_bounds__(kBuildBlock) void k_tile_scatter(const unsigned *__restrict__ tile_bucket,
                                                               unsigned tiles, unsigned *__restrict__ bucket_off,
                                                               unsigned *__restrict__ order) {
+    __shared__ unsigned count[kCostBuckets];
+    __shared__ unsigned base[kCostBuckets];
+    if (threadIdx.x < kCostBuckets)
+        count[threadIdx.x] = 0u;
+    __syncthreads();
     const unsigned t = blockIdx.x * kBuildBlock + threadIdx.x;
-    if (t >= tiles)
-        return;
-    order[atomicAdd(&bucket_off[tile_bucket[t]], 1u)] = t;
+    const unsigned bucket = t < tiles ? tile_bucket[t] : 0u;
+    const unsigned rank = t < tiles ? atomicAdd(&count[bucket], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x < kCostBuckets)
+        base[threadIdx.x] = count[threadIdx.x] ? atomicAdd(&bucket_off[threadIdx.x], count[threadIdx.x]) : 0u;
+    __syncthreads();
+    if (t < tiles)
+        order[base[bucket] + rank] = t;
 }
 
 /* Instance holding record r of the host order (binary search over rec). */
@@ -375,9 +395,9 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
     unsigned *dst = b->cols[b->cur ^ 1];
     const unsigned agent_blocks = blocks_for(n, kBuildBlock, 0xFFFFFFFFu);
 
-    k_batch_box<<<b->instances, kBuildBlock, 0, s>>>(src, cap, b->inst, b->dims, b->boxes);
-    k_batch_geometry<<<(b->instances + 63u) / 64u, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->instances, b->dims,
-                                                               b->geom);
+    k_tile_box<<<(b->tiles + kBuildBlock / 64u - 1u) / (kBuildBlock / 64u), kBuildBlock, 0, s>>>(
+        src, cap, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes);
+    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->geom);
     k_batch_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, b->tile_inst, b->inst, b->dims, b->geom,
                                                       b->keys[0]);
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
@@ -603,7 +623,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
               device_alloc((void **)&b->inst, sizeof(TayBatchInstance) * instances, "batch instances") &&
               device_alloc((void **)&b->tile_inst, 4ull * b->tiles, "tile instances") &&
               device_alloc((void **)&b->parts, 16ull * instances, "part sizes") &&
-              device_alloc((void **)&b->boxes, 24ull * instances, "boxes") &&
+              device_alloc((void **)&b->boxes, 24ull * b->tiles, "tile boxes") &&
               device_alloc((void **)&b->geom, sizeof(TayGridGeom) * instances, "geometry") &&
               device_alloc((void **)&b->keys[0], 4ull * slots, "sort keys") &&
               device_alloc((void **)&b->keys[1], 4ull * slots, "sort keys") &&

@@ -76,6 +76,7 @@ struct TayState {
     unsigned sources_count;
     TayGpuProgram *program;
     int count_pairs;
+    unsigned tile_width;
     int timing;
     int fetch_mode;
 };
@@ -228,6 +229,7 @@ int state_compile(TayState *state) {
     }
     if (state->timing)
         tay_gpu_program_enable_timing(state->program, 1);
+    tay_gpu_program_set_tile_width(state->program, state->tile_width);
     return 1;
 }
 
@@ -248,6 +250,7 @@ TAY_API TayState *tay_create_state() {
     state->sources_count = 0;
     state->program = nullptr;
     state->count_pairs = 0;
+    state->tile_width = 64;
     state->timing = 0;
     state->fetch_mode = TAY_FETCH_EVERY_RUN;
     return state;
@@ -553,6 +556,17 @@ TAY_API int tay_get_pair_count(TayState *state, unsigned pass_index, unsigned lo
         set_device_error(state, "pair count");
         return 0;
     }
+    return 1;
+}
+
+TAY_API int tay_set_tile_width(TayState *state, unsigned width) {
+    if (width != 64 && width != 32 && width != 16 && width != 8) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "tile width must be 64, 32, 16 or 8");
+        return 0;
+    }
+    state->tile_width = width;
+    if (state->program)
+        tay_gpu_program_set_tile_width(state->program, width);
     return 1;
 }
 
