@@ -198,6 +198,88 @@ def bench_ensemble(lib, args):
     return out
 
 
+def bench_sph(lib, args):
+    """SPH dam break: two see passes (density, forces) and two act passes
+    per step on 2M particles."""
+    from tay_amd import api
+    from tay_amd import workloads as W
+    recs, lo, hi = W.sph_particles(args.sph_particles, h=1.0)
+    ctx = W.sph_context(h=1.0, dt=1e-3, box_min=lo, box_max=hi)
+    model = W.sph_model(ctx, 1.0)
+    steps, warm = args.sph_steps, 2
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as twin:
+        lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
+        lib.tay_set_pair_counting(twin.state, 1)
+        twin.run(warm)
+        twin.run(steps)
+        pairs = [lib.pair_counts(twin.state, i)[1] for i in (0, 2)]
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
+        lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
+        sim.run(warm)
+        lib.tay_set_pass_timing(sim.state, 1)
+        lib.tay_gpu_synchronize()
+        t0 = time.perf_counter()
+        sim.run(steps)
+        lib.tay_gpu_synchronize()
+        wall = time.perf_counter() - t0
+        see = [lib.pass_time(sim.state, i)[0] for i in (0, 2)]
+        act = [lib.pass_time(sim.state, i)[0] for i in (1, 3)]
+    return {"workload": "SPH dam break, %d particles" % len(recs), "particles": len(recs), "h": 1.0, "dt": 1e-3,
+            "steps": steps, "p_see_per_step": [p / steps for p in pairs],
+            "pairs_per_s": sum(pairs) / (sum(see) / 1e3), "steps_per_s": steps / wall,
+            "ms_per_step": wall * 1e3 / steps, "see_ms_per_step": [t / steps for t in see],
+            "act_ms_per_step": [t / steps for t in act]}
+
+
+def bench_adjoint(lib, args):
+    """Differentiable flocking: 65,536 agents, 50 steps, checkpoints every
+    10; forward and backward device time at the same (FP32) accuracy, then
+    calibration of the three weights to a target trajectory made with hidden
+    weights."""
+    import numpy as np
+    from tay_amd import workloads as W
+    from tay_amd.dflock import DiffFlock
+    n, steps, every = args.adjoint_agents, 50, 10
+    p0, v0 = W.smooth_flock_agents(n, seed=34)
+    hidden = np.array([1.4, 0.7, 0.9], np.float32)
+    w = np.array([1.0, 1.0, 1.0])
+    out = {"workload": "differentiable flocking, %d agents" % n, "agents": n, "steps": steps,
+           "checkpoint_every": every, "radius": 20.0, "h2": 1.0, "dt": 0.1, "damping": 0.05}
+    with DiffFlock(lib, n, 20.0, 1.0, 0.1, 0.05, checkpoint_every=every) as f:
+        target, _ = f.forward(p0, v0, hidden, steps)
+        target = target.astype(np.float64)
+        fwd = bwd = 0.0
+        reps = 5
+        for r in range(reps + 1):
+            p, _ = f.forward(p0, v0, w.astype(np.float32), steps)
+            f.backward((2.0 * (p - target) / p.size).astype(np.float32))
+            fm, bm = f.times()
+            if r:
+                fwd += fm
+                bwd += bm
+        out.update({"forward_ms": fwd / reps, "backward_ms": bwd / reps, "backward_over_forward": bwd / fwd,
+                    "forward_steps_per_s": steps / (fwd / reps / 1e3), "max_neighbours": f.max_neighbours()})
+        m, s = np.zeros(3), np.zeros(3)
+        loss0 = None
+        t0 = time.perf_counter()
+        iters = args.adjoint_iters
+        for it in range(1, iters + 1):
+            p, _ = f.forward(p0, v0, w.astype(np.float32), steps)
+            e = p - target
+            if loss0 is None:
+                loss0 = float(np.mean(e * e))
+            _, _, g = f.backward((2.0 * e / p.size).astype(np.float32))
+            m = 0.9 * m + 0.1 * g
+            s = 0.999 * s + 0.001 * g * g
+            lr = 0.05 if it <= iters // 2 else 0.01
+            w -= lr * (m / (1 - 0.9 ** it)) / (np.sqrt(s / (1 - 0.999 ** it)) + 1e-12)
+        p, _ = f.forward(p0, v0, w.astype(np.float32), steps)
+        out["calibration"] = {"iterations": iters, "optimizer": "Adam on the 3 weights", "start": [1.0, 1.0, 1.0],
+                              "hidden": hidden.tolist(), "found": w.tolist(), "loss_start": loss0,
+                              "loss_end": float(np.mean((p - target) ** 2)), "seconds": time.perf_counter() - t0}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +293,12 @@ def main():
     ap.add_argument("--ensemble-instances", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--sph-particles", type=int, default=2_000_000)
+    ap.add_argument("--sph-steps", type=int, default=20)
+    ap.add_argument("--no-sph", action="store_true", help="skip the SPH measurement")
+    ap.add_argument("--adjoint-agents", type=int, default=65536)
+    ap.add_argument("--adjoint-iters", type=int, default=60)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-flocking measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one plain device-to-device copy of MIB MiB (reads and writes MIB MiB), the "
                          "calibration kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -307,6 +395,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(recs, model, args.cpu_threads, args.cpu_steps)
     if not args.no_ensemble:
         out["ensemble"] = bench_ensemble(lib, args)
+    if not args.no_sph:
+        out["sph"] = bench_sph(lib, args)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(lib, args)
     print(json.dumps(out))
     return 0
 

@@ -80,3 +80,40 @@ void flock_act(global FlockAgent *a, constant FlockActContext *c) {
     a->separation_count = 0;
     a->cohesion_count = 0;
 }
+
+void smooth_see(global SmoothAgent *a, global SmoothAgent *b, constant SmoothSeeContext *c) {
+    float dx = b->p.x - a->p.x, dy = b->p.y - a->p.y, dz = b->p.z - a->p.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < c->r2))
+        return;
+    float t = 1.0f - r2 * c->inv_r2;
+    float w = t * t * t;
+    float q = r2 + c->h2;
+    a->weight_sum += w;
+    a->alignment.x += w * (b->v.x - a->v.x);
+    a->cohesion.x += w * dx;
+    a->separation.x -= w * dx / q;
+    a->alignment.y += w * (b->v.y - a->v.y);
+    a->cohesion.y += w * dy;
+    a->separation.y -= w * dy / q;
+    a->alignment.z += w * (b->v.z - a->v.z);
+    a->cohesion.z += w * dz;
+    a->separation.z -= w * dz / q;
+}
+
+void smooth_act(global SmoothAgent *a, constant SmoothActContext *c) {
+    float D = a->weight_sum + 1.0f;
+    float ax = (c->separation_w * a->separation.x + c->alignment_w * a->alignment.x + c->cohesion_w * a->cohesion.x) / D;
+    float ay = (c->separation_w * a->separation.y + c->alignment_w * a->alignment.y + c->cohesion_w * a->cohesion.y) / D;
+    float az = (c->separation_w * a->separation.z + c->alignment_w * a->alignment.z + c->cohesion_w * a->cohesion.z) / D;
+    a->v.x = a->v.x + c->dt * (ax - c->damping * a->v.x);
+    a->v.y = a->v.y + c->dt * (ay - c->damping * a->v.y);
+    a->v.z = a->v.z + c->dt * (az - c->damping * a->v.z);
+    a->p.x = a->p.x + c->dt * a->v.x;
+    a->p.y = a->p.y + c->dt * a->v.y;
+    a->p.z = a->p.z + c->dt * a->v.z;
+    a->separation = float4_null();
+    a->alignment = float4_null();
+    a->cohesion = float4_null();
+    a->weight_sum = 0.0f;
+}

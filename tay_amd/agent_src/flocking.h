@@ -62,7 +62,41 @@ typedef struct __attribute__((packed)) FlockActContext {
     float max_speed;
 } FlockActContext;
 
+/* SmoothAgent / smooth_see / smooth_act: the differentiable variant
+ * (include/tay_dflock.h): cubic cutoff weight w = (1 - r^2 / R^2)^3 for
+ * r^2 < R^2, softened separation, no front filter and no speed clamp.
+ *   see: weight_sum += w, alignment += w (v_b - v_a), cohesion += w d,
+ *        separation -= w d / (r^2 + h2)                      (d = p_b - p_a)
+ *   act: a = (ws S + wa A + wc C) / (weight_sum + 1),
+ *        v += dt (a - damping v), p += dt v, clear the sums */
+typedef struct __attribute__((packed)) SmoothAgent {
+    TayAgentTag tag;
+    float4 p;
+    float4 v;
+    float4 separation;
+    float4 alignment;
+    float4 cohesion;
+    float weight_sum;
+    int result_index;
+} SmoothAgent;
+
+typedef struct __attribute__((packed)) SmoothSeeContext {
+    float r2;           /* R^2 */
+    float inv_r2;       /* 1 / R^2 */
+    float h2;
+} SmoothSeeContext;
+
+typedef struct __attribute__((packed)) SmoothActContext {
+    float separation_w;
+    float alignment_w;
+    float cohesion_w;
+    float dt;
+    float damping;
+} SmoothActContext;
+
 void boid_see(global Boid *a, global Boid *b, constant BoidSeeContext *c);
 void boid_act(global Boid *a, constant BoidActContext *c);
 void flock_see(global FlockAgent *a, global FlockAgent *b, constant FlockSeeContext *c);
 void flock_act(global FlockAgent *a, constant FlockActContext *c);
+void smooth_see(global SmoothAgent *a, global SmoothAgent *b, constant SmoothSeeContext *c);
+void smooth_act(global SmoothAgent *a, constant SmoothActContext *c);

@@ -40,6 +40,8 @@ BOID = np.dtype(TAG + [("p",) + _F4, ("dir",) + _F4, ("color", "<u4"), ("speed",
 FLOCK_AGENT = np.dtype(TAG + [("p",) + _F4, ("v",) + _F4, ("separation",) + _F4, ("alignment",) + _F4,
                               ("cohesion",) + _F4, ("separation_count", "<i4"), ("cohesion_count", "<i4"),
                               ("result_index", "<i4")])
+SMOOTH_AGENT = np.dtype(TAG + [("p",) + _F4, ("v",) + _F4, ("separation",) + _F4, ("alignment",) + _F4,
+                               ("cohesion",) + _F4, ("weight_sum", "<f4"), ("result_index", "<i4")])
 # m_sph/agent.h:2-11 layout
 SPH_PARTICLE = np.dtype(TAG + [("p",) + _F4, ("pressure_accum",) + _F4, ("viscosity_accum",) + _F4,
                                ("vh",) + _F4, ("v",) + _F4, ("density", "<f4"), ("pressure", "<f4")])
@@ -52,6 +54,9 @@ BOID_ACT_CONTEXT = np.dtype([("dummy", "<i4")])
 FLOCK_SEE_CONTEXT = np.dtype([("r", "<f4"), ("separation_r", "<f4")])
 FLOCK_ACT_CONTEXT = np.dtype([("separation_w", "<f4"), ("alignment_w", "<f4"), ("cohesion_w", "<f4"),
                               ("dt", "<f4"), ("max_speed", "<f4")])
+SMOOTH_SEE_CONTEXT = np.dtype([("r2", "<f4"), ("inv_r2", "<f4"), ("h2", "<f4")])
+SMOOTH_ACT_CONTEXT = np.dtype([("separation_w", "<f4"), ("alignment_w", "<f4"), ("cohesion_w", "<f4"),
+                               ("dt", "<f4"), ("damping", "<f4")])
 SPH_CONTEXT = np.dtype([("h", "<f4"), ("dynamic_viscosity", "<f4"), ("dt", "<f4"), ("m", "<f4"),
                         ("surface_tension", "<f4"), ("surface_tension_threshold", "<f4"), ("K", "<f4"),
                         ("density", "<f4"), ("min",) + _F4, ("max",) + _F4, ("h2", "<f4"), ("poly6", "<f4"),

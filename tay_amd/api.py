@@ -1,5 +1,5 @@
 """ctypes mirror of the Tay C API (include/tay.h, include/tay_ext.h,
-include/tay_gpu.h).
+include/tay_gpu.h, include/tay_dflock.h).
 
 TayLib binds one shared library that exports the reference API: the GPU
 library libtay_amd.so, or (in tests) the CPU oracle, which exports the same
@@ -62,6 +62,11 @@ class TayGpuPassShape(C.Structure):
 
 class TayGpuBatchPass(C.Structure):
     _fields_ = [("type", C.c_int), ("func_name", C.c_char_p), ("self_see", C.c_int), ("context_size", C.c_uint)]
+
+
+class TayDflockParams(C.Structure):
+    _fields_ = [("radius", C.c_float), ("h2", C.c_float), ("dt", C.c_float), ("damping", C.c_float),
+                ("cell_size", C.c_float)]
 
 
 _vp = C.c_void_p
@@ -146,6 +151,12 @@ _GPU = {
     "tay_gpu_batch_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_gpu_batch_pair_count": (_i, [_vp, _u, C.POINTER(C.c_ulonglong)]),
     "tay_gpu_batch_reset_pair_counts": (_i, [_vp]),
+    "tay_dflock_create": (_vp, [_u, C.POINTER(TayDflockParams), _u, C.c_ulonglong]),
+    "tay_dflock_destroy": (None, [_vp]),
+    "tay_dflock_forward": (_i, [_vp, _vp, _vp, _vp, _u, _vp, _vp]),
+    "tay_dflock_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "tay_dflock_times": (_i, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "tay_dflock_max_neighbours": (_i, [_vp, C.POINTER(C.c_uint)]),
     "tay_gpu_batch_autotune": (_i, [_vp, C.POINTER(C.c_float), _u, C.POINTER(C.c_uint), _u, _u,
                                     C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint)]),
 }

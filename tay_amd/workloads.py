@@ -78,6 +78,13 @@ def flock_agents(n, seed=1, clusters=64, sigma=30.0, box=2000.0):
     return rec
 
 
+def smooth_flock_agents(n, seed=1, clusters=16, sigma=30.0, box=500.0):
+    """Differentiable-flocking inputs: (positions, velocities) as (n, 3)
+    float32 arrays, Gaussian clusters and unit-random velocities."""
+    g = rng(seed)
+    return clustered_points(n, clusters, sigma, box, g), unit_vectors(n, g)
+
+
 def boids(n, seed=1, half=300.0):
     """m_flocking's boids (m_flocking/main.c:161-194): positions U[-half,
     half)^3, unit headings, speed 1."""
@@ -178,6 +185,21 @@ def flock_model(radius=25.0, weights=(1.5, 1.0, 1.0), dt=0.1, max_speed=2.0, par
     act["dt"] = dt
     act["max_speed"] = max_speed
     return Model(A.FLOCK_AGENT, [Pass("see", "flock_see", see, radius), Pass("act", "flock_act", act)],
+                 part=radius if part is None else part)
+
+
+def smooth_model(radius=20.0, weights=(1.0, 1.0, 1.0), h2=1.0, dt=0.1, damping=0.05, part=None):
+    """The differentiable flocking model (smooth_see / smooth_act)."""
+    see = np.zeros(1, A.SMOOTH_SEE_CONTEXT)
+    r = np.float32(radius)
+    see["r2"] = r * r
+    see["inv_r2"] = np.float32(1.0) / (r * r)
+    see["h2"] = h2
+    act = np.zeros(1, A.SMOOTH_ACT_CONTEXT)
+    act["separation_w"], act["alignment_w"], act["cohesion_w"] = weights
+    act["dt"] = dt
+    act["damping"] = damping
+    return Model(A.SMOOTH_AGENT, [Pass("see", "smooth_see", see, radius), Pass("act", "smooth_act", act)],
                  part=radius if part is None else part)
 
 

@@ -21,7 +21,7 @@ REFERENCE = "/root/reference"
 
 def declared_functions():
     names = set()
-    for header in ("tay.h", "tay_gpu.h", "tay_ext.h"):
+    for header in ("tay.h", "tay_gpu.h", "tay_ext.h", "tay_dflock.h"):
         text = open(os.path.join(INCLUDE, header)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"#define[^\n]*(\\\n[^\n]*)*", "", text)
