@@ -206,6 +206,15 @@ int tay_gpu_batch_set_contexts(TayGpuBatch *batch, unsigned pass_index, const vo
 /* seers per lane group at the start of a see tile: 64, 32, 16 or 8 */
 int tay_gpu_batch_set_tile_width(TayGpuBatch *batch, unsigned width);
 
+/* The order in which the persistent see kernels take tiles: estimated cost,
+ * heaviest first (the default), or storage order (neighbouring tiles share
+ * seen cells in cache).  Results do not depend on it. */
+typedef enum TayGpuTileOrder {
+    TAY_GPU_TILE_ORDER_COST = 0,
+    TAY_GPU_TILE_ORDER_STORAGE = 1,
+} TayGpuTileOrder;
+int tay_gpu_batch_set_tile_order(TayGpuBatch *batch, int order);
+
 /* All instances' agents as AoS records, instance after instance (sum of
  * counts records). */
 int tay_gpu_batch_push(TayGpuBatch *batch, const void *agents);

@@ -43,6 +43,8 @@ TAY_GPU_KIND_Z_GRID = 1
 TAY_GPU_KIND_SIMPLE = 2
 TAY_GPU_PASS_SEE = 0
 TAY_GPU_PASS_ACT = 1
+TAY_GPU_TILE_ORDER_COST = 0
+TAY_GPU_TILE_ORDER_STORAGE = 1
 
 
 class float4(C.Structure):
@@ -144,6 +146,7 @@ _GPU = {
     "tay_gpu_batch_set_radii": (_i, [_vp, _u, C.POINTER(C.c_float)]),
     "tay_gpu_batch_set_contexts": (_i, [_vp, _u, _vp]),
     "tay_gpu_batch_set_tile_width": (_i, [_vp, _u]),
+    "tay_gpu_batch_set_tile_order": (_i, [_vp, _i]),
     "tay_gpu_batch_push": (_i, [_vp, _vp]),
     "tay_gpu_batch_fetch": (_i, [_vp, _vp]),
     "tay_gpu_batch_run": (_i, [_vp, _u, _i, C.POINTER(C.c_double)]),

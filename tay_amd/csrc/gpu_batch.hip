@@ -256,6 +256,16 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_scatter(const unsigned *__
         order[base[bucket] + rank] = t;
 }
 
+/* Storage order: tile k is processed k-th; also clears the work queues. */
+__global__ __launch_bounds__(kBuildBlock) void k_tile_identity(unsigned tiles, unsigned *__restrict__ order,
+                                                               unsigned *__restrict__ queues, unsigned queues_count) {
+    const unsigned t = blockIdx.x * kBuildBlock + threadIdx.x;
+    if (t < tiles)
+        order[t] = t;
+    if (t < queues_count)
+        queues[t] = 0u;
+}
+
 /* Instance holding record r of the host order (binary search over rec). */
 __device__ __forceinline__ unsigned instance_of_record(const TayBatchInstance *__restrict__ inst, unsigned instances,
                                                        unsigned r) {
@@ -318,6 +328,7 @@ struct TayGpuBatch {
     int count_pairs;
     int timing;
     unsigned width;
+    int order_mode;             /* TayGpuTileOrder */
     unsigned radix_passes;
     std::vector<TayBatchInstance> inst_host;
     std::vector<float> radius_host;     /* first see pass's radii, 4 per instance */
@@ -425,10 +436,15 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
     k_batch_permute<<<agent_blocks, kBuildBlock, 0, s>>>(b->keys[f], b->vals[f], n, b->words, cap, b->tile_inst,
                                                          b->inst, b->dims, b->geom, b->ranges, src, dst);
     const unsigned tile_blocks = blocks_for(b->tiles, kBuildBlock, 0xFFFFFFFFu);
-    k_tile_cost<<<tile_blocks, kBuildBlock, 0, s>>>(b->keys[f], b->tiles, b->tile_inst, b->inst, b->dims, b->geom,
-                                                    b->ranges, b->tile_bucket, b->bucket_hist);
-    k_tile_scan<<<1, 64, 0, s>>>(b->bucket_hist, b->bucket_off, b->queues, (unsigned)b->passes.size());
-    k_tile_scatter<<<tile_blocks, kBuildBlock, 0, s>>>(b->tile_bucket, b->tiles, b->bucket_off, b->order);
+    if (b->order_mode == TAY_GPU_TILE_ORDER_COST) {
+        k_tile_cost<<<tile_blocks, kBuildBlock, 0, s>>>(b->keys[f], b->tiles, b->tile_inst, b->inst, b->dims,
+                                                        b->geom, b->ranges, b->tile_bucket, b->bucket_hist);
+        k_tile_scan<<<1, 64, 0, s>>>(b->bucket_hist, b->bucket_off, b->queues, (unsigned)b->passes.size());
+        k_tile_scatter<<<tile_blocks, kBuildBlock, 0, s>>>(b->tile_bucket, b->tiles, b->bucket_off, b->order);
+    } else {
+        k_tile_identity<<<tile_blocks, kBuildBlock, 0, s>>>(b->tiles, b->order, b->queues,
+                                                            (unsigned)b->passes.size());
+    }
     if (!launch_ok("batch sort"))
         return false;
     b->cur ^= 1;
@@ -588,6 +604,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     b->count_pairs = count_pairs;
     b->timing = 0;
     b->width = 64;
+    b->order_mode = TAY_GPU_TILE_ORDER_COST;
     b->radix_passes = kRadixPasses + (instances > 256 ? 2u : 1u);
     b->inst_host = inst;
     b->radius_host.assign(4ull * instances, 0.0f);
@@ -766,6 +783,17 @@ TAY_API int tay_gpu_batch_set_tile_width(TayGpuBatch *b, unsigned width) {
     if (width != b->width)
         drop_graph(b);
     b->width = width;
+    return 1;
+}
+
+TAY_API int tay_gpu_batch_set_tile_order(TayGpuBatch *b, int order) {
+    if (order != TAY_GPU_TILE_ORDER_COST && order != TAY_GPU_TILE_ORDER_STORAGE) {
+        set_error("unknown tile order %d", order);
+        return 0;
+    }
+    if (order != b->order_mode)
+        drop_graph(b);
+    b->order_mode = order;
     return 1;
 }
 

@@ -141,6 +141,22 @@ __device__ __forceinline__ void tay_wave_sync(void) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* Raw buffer resource over one column of cap words: loads through it take a
+ * 32-bit byte offset, so a lane's address costs one shift, not a 64-bit add
+ * per column. */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tay_column(const unsigned *base, unsigned cap) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(cap * 4u), 0x00020000);
+}
+
+__device__ __forceinline__ unsigned tay_wave_max_u(unsigned v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 /* One tile's view of the storage it reads and writes: the plain kernel has
  * one view for all tiles, a batched kernel one per instance. */
 struct TaySeeView {
@@ -207,6 +223,10 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     }
     tay_ctx_holder<typename T::context, CTX_WORDS> ctx_holder(v.context);
     typename T::context *ctx = ctx_holder.get();
+    const char *seen_col[SEEN_WORDS];
+#pragma unroll
+    for (unsigned k = 0; k < SEEN_WORDS; ++k)
+        seen_col[k] = (const char *)(v.seen_cols + (unsigned long long)k * v.seen_cap);
     unsigned pairs = 0;
     bool member = active;   /* lane belongs to the lane group being processed */
     unsigned fill = 0;      /* slots used in the current chunk (uniform) */
@@ -229,12 +249,15 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
 
     auto process = [&]() {
         if (lane < fill) {
+            const unsigned off = cand * 4u;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                chunk_lo[wave][lane][k] = k < DIMS ? __uint_as_float(v.seen_cols[(2u + k) * v.seen_cap + cand]) : 0.0f;
+                chunk_lo[wave][lane][k] =
+                    k < DIMS ? __uint_as_float(*(const unsigned *)(seen_col[2 + k] + off)) : 0.0f;
                 if (SEEN_BOX)
                     chunk_hi[wave][lane][k] =
-                        k < DIMS ? __uint_as_float(v.seen_cols[(6u + k) * v.seen_cap + cand]) : 0.0f;
+                        k < DIMS ? __uint_as_float(*(const unsigned *)(seen_col[6 + k] + off))
+                                 : 0.0f;
             }
             chunk_idx[wave][lane] = cand;
         }
@@ -246,11 +269,9 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         if (COUNT)
             pairs += (unsigned)__popcll(mask);
 
-        for (;;) {
-            const bool more = mask != 0ull;
-            if (!__any(more))
-                break;
-            if (more) {
+        const unsigned rounds = tay_wave_max_u((unsigned)__popcll(mask));
+        for (unsigned round = 0; round < rounds; ++round) {
+            if (mask != 0ull) {
                 const unsigned c = (unsigned)__builtin_ctzll(mask);
                 mask &= mask - 1ull;
                 const unsigned j = chunk_idx[wave][c];
@@ -258,9 +279,10 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                     typename T::seen agent;
                     unsigned w[SEEN_WORDS];
                 } B;
+                const unsigned off = j * 4u;
 #pragma unroll
                 for (unsigned k = 0; k < SEEN_WORDS; ++k)
-                    B.w[k] = v.seen_cols[k * v.seen_cap + j];
+                    B.w[k] = *(const unsigned *)(seen_col[k] + off);
 #pragma unroll
                 for (int k = 0; k < DIMS; ++k)
                     B.w[2 + k] = __float_as_uint(chunk_lo[wave][c][k]);
