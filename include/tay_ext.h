@@ -32,6 +32,15 @@ int tay_get_pair_count(TayState *state, unsigned pass_index, unsigned long long 
 void tay_set_pass_timing(TayState *state, int enable);
 int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, unsigned *launches);
 
+/* Order of a Z-grid group's agents inside each cell: refined by `bits`
+ * sub-cell bits per axis (0 to 3; default 2), i.e. by the Morton order of a
+ * 2^bits-finer grid, then by previous storage order; 0 keeps the previous
+ * storage order alone, as the reference's CPU grids do.  The finest order
+ * whose key fits the Morton range is used each step.  It changes the order
+ * in which a seer meets its pairs (not the pairs), so float sums differ in
+ * rounding between settings. */
+int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
+
 /* Seers per lane group when a see tile starts (64, 32, 16 or 8; default
  * 64), as tay_gpu_program_set_tile_width; results do not depend on it. */
 int tay_set_tile_width(TayState *state, unsigned width);

@@ -63,6 +63,16 @@ void tay_gpu_space_destroy(TayGpuSpace *space);
 int tay_gpu_space_fix_box(TayGpuSpace *space, const float *min, const float *max);
 int tay_gpu_space_unfix_box(TayGpuSpace *space);
 
+/* Sub-cell bits (0 to 3, default 2) of the Z-grid's storage order inside a
+ * cell: agents of a cell are ordered by the Morton key of a 2^bits-finer
+ * grid, then by previous storage order, so the 64 seers of a wavefront and
+ * each 64-candidate chunk are spatially compact.  0 keeps the previous
+ * storage order alone (the reference's CPU grids, tay/cpu_grid.c:114-150).
+ * The reference's OpenCL grid ranks agents in a cell by atomic_inc
+ * (tay/ocl_grid.c:147): no fixed order at all.  No reference counterpart as
+ * a setting. */
+int tay_gpu_space_set_order_bits(TayGpuSpace *space, int bits);
+
 /* Copies count AoS records from host memory to the device and makes them the
  * space's agents.  Replaces ocl_push_agents_non_blocking (tay/ocl.c:238-250). */
 int tay_gpu_space_push(TayGpuSpace *space, const void *agents, unsigned count);
@@ -214,6 +224,10 @@ typedef enum TayGpuTileOrder {
     TAY_GPU_TILE_ORDER_STORAGE = 1,
 } TayGpuTileOrder;
 int tay_gpu_batch_set_tile_order(TayGpuBatch *batch, int order);
+
+/* Sub-cell bits of every instance's storage order, as
+ * tay_gpu_space_set_order_bits. */
+int tay_gpu_batch_set_order_bits(TayGpuBatch *batch, int bits);
 
 /* All instances' agents as AoS records, instance after instance (sum of
  * counts records). */

@@ -102,6 +102,8 @@ typedef struct {
     CellRange *ranges;
     unsigned long long gz_max_cells;
     uint64_t *sort_buf;     /* (key << 32 | index) pairs */
+    int order_bits;         /* requested sub-cell bits of the storage order */
+    int gz_order_bits;      /* the ones used by the last sort */
 } Space;
 
 typedef struct TayGroup {
@@ -330,6 +332,7 @@ TayGroup *tay_add_group(TayState *state, unsigned agent_size, unsigned agent_cap
     g->space.type = TAY_CPU_SIMPLE;
     g->space.dims = 3;
     g->space.min_part_sizes = (float4){{{1.0f, 1.0f, 1.0f, 1.0f}}};
+    g->space.order_bits = 2;   /* the GPU library's default */
     return g;
 }
 
@@ -920,13 +923,26 @@ static void cpu_z_grid_see_seen(TayPass *p, Slice seer, Box box, int dims, PairC
 }
 
 /* ------------------------------------------------- GPU Z-grid statement */
+/* Sub-cell bits of the GPU Z-grid's storage order (the GPU library's
+ * tay_set_order_bits, same default). */
+int tay_set_order_bits(TayState *state, TayGroup *group, int bits) {
+    if (bits < 0 || bits > 3) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "order bits must be 0 to 3");
+        return 0;
+    }
+    group->space.order_bits = bits;
+    return 1;
+}
+
 /* The geometry is the reference OpenCL Z-grid's (tay/ocl_grid.c:128-136):
  * count_i = ceil(size_i / part_i), cell_size_i = size_i / count_i with
  * size_i = max_i - min_i + 0.001 part_i as in every reference grid.  That
  * kernel has no memory cap; this space coarsens by the smallest integer k with
  * ceil(count_i / k) fitting both the Morton range (1024 per axis in 3D, 65536
- * in 2D) and shared_size / 8 cells.  Agents are ordered by Morton key,
- * ties kept in previous storage order; part_i is the key and cell_agent_i the
+ * in 2D) and shared_size / 8 cells.  Agents are ordered by a storage-order
+ * key: the cell's Morton key refined by b sub-cell bits per axis (the
+ * Morton key of a 2^b-finer grid, b the finest of order_bits that fits the
+ * Morton range), ties kept in previous storage order; part_i is the key and cell_agent_i the
  * rank in the cell.  Each seer visits the cells overlapping
  * [p - r - m, p + r + m] (m = (|p| + r) * 2^-20 covers the rounding of
  * p - r and of the box test) x outermost, then y, z, and inside a cell the
@@ -976,6 +992,17 @@ static void gpu_z_grid_geometry(Space *s) {
         if (i < s->dims)
             s->cells_count *= (unsigned long long)counts[i];
     }
+    /* the finest sub-cell order whose key fits the Morton range */
+    int b = s->order_bits;
+    for (; b > 0; --b) {
+        int fits = 1;
+        for (int i = 0; i < s->dims; ++i)
+            if (((unsigned long long)counts[i] << b) > limit)
+                fits = 0;
+        if (fits)
+            break;
+    }
+    s->gz_order_bits = b;
 }
 
 static int cmp_u64(const void *a, const void *b) {
@@ -988,13 +1015,29 @@ static void gpu_z_grid_sort(TayGroup *g) {
     update_box(g);
     gpu_z_grid_geometry(s);
 
+    /* Storage-order key: per axis q = (p - o) / cs, the cell c = floorf(q)
+     * (clamped: only a fixed box can leave agents outside) refined by
+     * b sub-cell bits, sub = floorf((q - c) * 2^b) clamped to [0, 2^b - 1];
+     * the key is the Morton interleave of (c << b) | sub, whose top bits are
+     * the cell's Morton key.  Ties keep the previous storage order. */
+    const int b = s->gz_order_bits;
+    const unsigned shift = (unsigned)(s->dims * b);
     s->sort_buf = (uint64_t *)realloc(s->sort_buf, sizeof(uint64_t) * (s->count ? s->count : 1));
     for (unsigned a = 0; a < s->count; ++a) {
-        const char *agent = g->storage + (size_t)g->agent_size * a;
-        Int4 idx = cell_indices(POS(agent), s->origin, s->cell_sizes, s->dims);
-        for (int k = 0; k < s->dims; ++k) {   /* only a fixed box can leave agents outside */
-            if (idx.arr[k] < 0) idx.arr[k] = 0;
-            if (idx.arr[k] >= s->cell_counts.arr[k]) idx.arr[k] = s->cell_counts.arr[k] - 1;
+        const float *p = POS(g->storage + (size_t)g->agent_size * a);
+        Int4 idx = {{0, 0, 0, 0}};
+        for (int k = 0; k < s->dims; ++k) {
+            float q = (p[k] - s->origin.arr[k]) / s->cell_sizes.arr[k];
+            int c = (int)floorf(q);
+            if (c < 0) c = 0;
+            if (c >= s->cell_counts.arr[k]) c = s->cell_counts.arr[k] - 1;
+            int sub = 0;
+            if (b) {
+                sub = (int)floorf((q - (float)c) * (float)(1 << b));
+                if (sub < 0) sub = 0;
+                if (sub > (1 << b) - 1) sub = (1 << b) - 1;
+            }
+            idx.arr[k] = (c << b) | sub;
         }
         s->sort_buf[a] = ((uint64_t)morton(idx, s->dims) << 32) | a;
     }
@@ -1004,15 +1047,15 @@ static void gpu_z_grid_sort(TayGroup *g) {
         s->ranges[ci].beg = s->ranges[ci].end = 0;
     unsigned run_beg = 0;
     for (unsigned i = 0; i < s->count; ++i) {
-        unsigned key = (unsigned)(s->sort_buf[i] >> 32);
+        unsigned key = (unsigned)(s->sort_buf[i] >> 32) >> shift;
         unsigned src = (unsigned)(s->sort_buf[i] & 0xFFFFFFFFu);
-        if (i > 0 && key != (unsigned)(s->sort_buf[i - 1] >> 32))
+        if (i > 0 && key != (unsigned)(s->sort_buf[i - 1] >> 32) >> shift)
             run_beg = i;
         TayAgentTag *dst = (TayAgentTag *)(g->sort_storage + (size_t)g->agent_size * i);
         memcpy(dst, g->storage + (size_t)g->agent_size * src, (size_t)g->agent_size);
         dst->part_i = key;
         dst->cell_agent_i = i - run_beg;
-        if (i + 1 == s->count || key != (unsigned)(s->sort_buf[i + 1] >> 32)) {
+        if (i + 1 == s->count || key != (unsigned)(s->sort_buf[i + 1] >> 32) >> shift) {
             int x, y, z;
             oracle_morton_decode(key, s->dims, &x, &y, &z);
             Int4 idx = {{x, y, z, 0}};

@@ -42,6 +42,10 @@ void oracle_get_grid_info(TayState *state, TayGroup *group,
 unsigned oracle_morton_key(int x, int y, int z, int dims);
 void oracle_morton_decode(unsigned key, int dims, int *x, int *y, int *z);
 
+/* Sub-cell bits (0 to 3, default 2) of the GPU Z-grid statement's storage
+ * order inside a cell; the GPU library exports the same call. */
+int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
+
 /* Wall-clock seconds spent in see passes during the last tay_run. */
 double oracle_get_see_seconds(TayState *state);
 

@@ -110,6 +110,7 @@ _GPU = {
     "tay_get_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_get_gpu_space": (_vp, [_vp, _vp]),
     "tay_set_tile_width": (_i, [_vp, _u]),
+    "tay_set_order_bits": (_i, [_vp, _vp, _i]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),
     "tay_gpu_available": (_i, []),
@@ -120,6 +121,7 @@ _GPU = {
     "tay_gpu_space_destroy": (None, [_vp]),
     "tay_gpu_space_fix_box": (_i, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "tay_gpu_space_unfix_box": (_i, [_vp]),
+    "tay_gpu_space_set_order_bits": (_i, [_vp, _i]),
     "tay_gpu_space_push": (_i, [_vp, _vp, _u]),
     "tay_gpu_space_fetch": (_i, [_vp, _vp, _u]),
     "tay_gpu_space_count": (_u, [_vp]),
@@ -147,6 +149,7 @@ _GPU = {
     "tay_gpu_batch_set_contexts": (_i, [_vp, _u, _vp]),
     "tay_gpu_batch_set_tile_width": (_i, [_vp, _u]),
     "tay_gpu_batch_set_tile_order": (_i, [_vp, _i]),
+    "tay_gpu_batch_set_order_bits": (_i, [_vp, _i]),
     "tay_gpu_batch_push": (_i, [_vp, _vp]),
     "tay_gpu_batch_fetch": (_i, [_vp, _vp]),
     "tay_gpu_batch_run": (_i, [_vp, _u, _i, C.POINTER(C.c_double)]),

@@ -16,6 +16,7 @@ struct TayGridGeom {
     unsigned long long cells;   /* product of count over the used axes */
     float box_min[4];
     float box_max[4];
+    int order_bits;             /* sub-cell bits per axis of the storage-order key */
 };
 
 /* One cell's agents are [beg, end) of the key-sorted storage. */

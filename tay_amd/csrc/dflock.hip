@@ -355,8 +355,8 @@ void df_sort(TayDflock *f, const float *cols, hipStream_t s) {
     const unsigned box_blocks = blocks_for(n, kBuildBlock * 4, f->box_blocks);
     const float part = f->params.cell_size;
     k_box_partial<<<box_blocks, kBuildBlock, 0, s>>>(c, n, n, 3, 1, f->box_partials);
-    k_geometry<<<1, 64, 0, s>>>(f->box_partials, box_blocks, 3, part, part, part, f->max_cells, 0, 0, 0, 0, 0, 0,
-                                0, f->geom);
+    k_geometry<<<1, 64, 0, s>>>(f->box_partials, box_blocks, 3, part, part, part, f->max_cells,
+                                TAY_GPU_DEFAULT_ORDER_BITS, 0, 0, 0, 0, 0, 0, 0, f->geom);
     const unsigned agent_blocks = blocks_for(n, kBuildBlock, 0xFFFFFFFFu);
     k_keys<<<agent_blocks, kBuildBlock, 0, s>>>(c, n, n, 3, f->geom, f->keys[0]);
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;

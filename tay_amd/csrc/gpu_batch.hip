@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_box(const unsigned *__rest
 __global__ __launch_bounds__(64) void k_batch_geometry(const float *__restrict__ partials,
                                                        const float *__restrict__ parts,
                                                        const TayBatchInstance *__restrict__ inst, int dims,
-                                                       TayGridGeom *__restrict__ geom) {
+                                                       int order_bits, TayGridGeom *__restrict__ geom) {
     const unsigned b = blockIdx.x, lane = threadIdx.x;
     const TayBatchInstance in = inst[b];
     const unsigned t0 = in.off / 64u, t1 = (in.off + in.count + 63u) / 64u;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void k_batch_geometry(const float *__restrict__
         if (in.count == 0)
             lo[k] = hi[k] = 0.0f;   /* no agents: a one-cell grid that no tile reads */
     }
-    grid_geometry(lo, hi, part, dims, in.max_cells, geom + b);
+    grid_geometry(lo, hi, part, dims, in.max_cells, order_bits, geom + b);
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_batch_keys(const unsigned *__restrict__ cols, unsigned cap,
@@ -113,13 +113,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_batch_keys(const unsigned *__re
         keys[i] = 0xFFFFFFFFu;   /* padding sorts after every cell of its instance */
         return;
     }
-    const TayGridGeom *g = geom + b;
-    int idx[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (k < dims)
-            idx[k] = cell_axis(__uint_as_float(cols[(2u + k) * cap + i]), g->origin[k], g->cell_size[k], g->count[k]);
-    keys[i] = morton(idx[0], idx[1], idx[2], dims);
+    keys[i] = storage_key(cols, cap, i, dims, geom + b);
 }
 
 /* blockIdx.y is the instance; its blocks clear its table's used cells. */
@@ -149,11 +143,12 @@ __global__ __launch_bounds__(kBuildBlock) void k_batch_ranges(const unsigned *__
     const unsigned li = i - in.off;
     if (li >= in.count)
         return;
-    const unsigned key = keys[i];
-    const bool first = li == 0 || keys[i - 1] != key;
-    const bool last = li + 1 == in.count || keys[i + 1] != key;
+    const TayGridGeom *g = geom + b;
+    const unsigned key = cell_of_key(keys[i], dims, g);
+    const bool first = li == 0 || cell_of_key(keys[i - 1], dims, g) != key;
+    const bool last = li + 1 == in.count || cell_of_key(keys[i + 1], dims, g) != key;
     if (first || last) {
-        TayCellRange *r = ranges + in.ranges_off + key_to_cell(key, dims, geom + b);
+        TayCellRange *r = ranges + in.ranges_off + morton_to_cell(key, dims, g);
         if (first)
             r->beg = i;
         if (last)
@@ -180,7 +175,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_batch_permute(const unsigned *_
     const unsigned key = keys[i];
     const unsigned from = vals[i];
     const bool real = i - in.off < in.count;
-    dst[i] = key;
+    dst[i] = real ? cell_of_key(key, dims, geom + b) : key;
     dst[cap + i] = real ? i - ranges[in.ranges_off + key_to_cell(key, dims, geom + b)].beg : 0u;
     for (unsigned w = 2; w < words; ++w)
         dst[w * cap + i] = src[w * cap + from];
@@ -329,6 +324,7 @@ struct TayGpuBatch {
     int timing;
     unsigned width;
     int order_mode;             /* TayGpuTileOrder */
+    int order_bits;             /* sub-cell bits of the storage order */
     unsigned radix_passes;
     std::vector<TayBatchInstance> inst_host;
     std::vector<float> radius_host;     /* first see pass's radii, 4 per instance */
@@ -408,7 +404,7 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
 
     k_tile_box<<<(b->tiles + kBuildBlock / 64u - 1u) / (kBuildBlock / 64u), kBuildBlock, 0, s>>>(
         src, cap, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes);
-    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->geom);
+    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom);
     k_batch_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, b->tile_inst, b->inst, b->dims, b->geom,
                                                       b->keys[0]);
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
@@ -605,6 +601,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     b->timing = 0;
     b->width = 64;
     b->order_mode = TAY_GPU_TILE_ORDER_COST;
+    b->order_bits = TAY_GPU_DEFAULT_ORDER_BITS;
     b->radix_passes = kRadixPasses + (instances > 256 ? 2u : 1u);
     b->inst_host = inst;
     b->radius_host.assign(4ull * instances, 0.0f);
@@ -794,6 +791,17 @@ TAY_API int tay_gpu_batch_set_tile_order(TayGpuBatch *b, int order) {
     if (order != b->order_mode)
         drop_graph(b);
     b->order_mode = order;
+    return 1;
+}
+
+TAY_API int tay_gpu_batch_set_order_bits(TayGpuBatch *b, int bits) {
+    if (bits < 0 || bits > 3) {
+        set_error("order bits %d: must be 0 to 3", bits);
+        return 0;
+    }
+    if (bits != b->order_bits)
+        drop_graph(b);
+    b->order_bits = bits;
     return 1;
 }
 

@@ -171,6 +171,7 @@ TAY_API TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigne
     for (int k = 0; k < 4; ++k)
         sp->part[k] = part_sizes ? part_sizes[k] : 1.0f;
     sp->shared_bytes = shared_bytes;
+    sp->order_bits = TAY_GPU_DEFAULT_ORDER_BITS;
 
     const size_t col_bytes = (size_t)sp->words * sp->capacity * 4;
     bool ok = device_alloc((void **)&sp->cols[0], col_bytes, "agent columns") &&
@@ -214,6 +215,15 @@ TAY_API int tay_gpu_space_fix_box(TayGpuSpace *sp, const float *min, const float
         sp->fixed_max[k] = max[k];
     }
     sp->box_fixed = 1;
+    return 1;
+}
+
+TAY_API int tay_gpu_space_set_order_bits(TayGpuSpace *sp, int bits) {
+    if (bits < 0 || bits > 3) {
+        set_error("order bits %d: must be 0 to 3", bits);
+        return 0;
+    }
+    sp->order_bits = bits;
     return 1;
 }
 
@@ -290,7 +300,7 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
     const unsigned box_blocks = blocks_for(n, kBuildBlock * 4, sp->box_blocks);
     k_box_partial<<<box_blocks, kBuildBlock, 0, s>>>(src, cap, n, sp->dims, sp->is_point, sp->box_partials);
     k_geometry<<<1, 64, 0, s>>>(sp->box_partials, box_blocks, sp->dims, sp->part[0], sp->part[1], sp->part[2],
-                                sp->max_cells, sp->box_fixed, sp->fixed_min[0], sp->fixed_min[1],
+                                sp->max_cells, sp->order_bits, sp->box_fixed, sp->fixed_min[0], sp->fixed_min[1],
                                 sp->fixed_min[2], sp->fixed_max[0], sp->fixed_max[1], sp->fixed_max[2], sp->geom);
     const unsigned agent_blocks = blocks_for(n, kBuildBlock, 0xFFFFFFFFu);
     k_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, sp->dims, sp->geom, sp->keys[0]);

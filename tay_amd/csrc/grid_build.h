@@ -70,8 +70,15 @@ __device__ __forceinline__ unsigned morton(int x, int y, int z, int dims) {
     return (unsigned)x;
 }
 
-/* Row-major cell id (x fastest) of a Morton key. */
-__device__ __forceinline__ unsigned long long key_to_cell(unsigned key, int dims, const TayGridGeom *g) {
+/* Cell part of a storage-order key: the key interleaves (cell << s) | sub on
+ * every axis, so the cell's Morton key is the key without its low dims x s
+ * bits. */
+__device__ __forceinline__ unsigned cell_of_key(unsigned key, int dims, const TayGridGeom *g) {
+    return key == 0xFFFFFFFFu ? key : key >> ((unsigned)dims * (unsigned)g->order_bits);
+}
+
+/* Row-major cell id (x fastest) of a cell's Morton key. */
+__device__ __forceinline__ unsigned long long morton_to_cell(unsigned key, int dims, const TayGridGeom *g) {
     unsigned x, y = 0, z = 0;
     if (dims == 3) {
         x = compact3(key);
@@ -85,6 +92,11 @@ __device__ __forceinline__ unsigned long long key_to_cell(unsigned key, int dims
     }
     return (unsigned long long)x + (unsigned long long)y * (unsigned)g->count[0] +
            (unsigned long long)z * ((unsigned long long)(unsigned)g->count[0] * (unsigned)g->count[1]);
+}
+
+/* Row-major cell id of a storage-order key. */
+__device__ __forceinline__ unsigned long long key_to_cell(unsigned key, int dims, const TayGridGeom *g) {
+    return morton_to_cell(cell_of_key(key, dims, g), dims, g);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -144,7 +156,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_box_partial(const unsigned *__r
  * Morton range and their product within max_cells; then
  *   cell_size = size / (float)count. */
 __device__ void grid_geometry(const float *lo, const float *hi, const float *part, int dims,
-                              unsigned long long max_cells, TayGridGeom *__restrict__ geom) {
+                              unsigned long long max_cells, int order_bits, TayGridGeom *__restrict__ geom) {
     float size[3] = {1.0f, 1.0f, 1.0f};
     unsigned long long count[3] = {1, 1, 1};
     for (int k = 0; k < dims; ++k) {
@@ -194,13 +206,23 @@ __device__ void grid_geometry(const float *lo, const float *hi, const float *par
         }
     }
     geom->cells = cells;
+    /* the finest sub-cell order whose key still fits the Morton range */
+    int s = order_bits;
+    for (; s > 0; --s) {
+        bool fits = true;
+        for (int a = 0; a < dims; ++a)
+            fits = fits && ((unsigned long long)geom->count[a] << s) <= limit;
+        if (fits)
+            break;
+    }
+    geom->order_bits = s;
 }
 
 /* One wavefront: reduce the block boxes and derive the grid. */
 __global__ __launch_bounds__(64) void k_geometry(const float *__restrict__ partials, unsigned nparts, int dims,
                                                  float px, float py, float pz, unsigned long long max_cells,
-                                                 int box_fixed, float fx0, float fy0, float fz0, float fx1,
-                                                 float fy1, float fz1, TayGridGeom *__restrict__ geom) {
+                                                 int order_bits, int box_fixed, float fx0, float fy0, float fz0,
+                                                 float fx1, float fy1, float fz1, TayGridGeom *__restrict__ geom) {
     const unsigned lane = threadIdx.x;
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
     float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -223,7 +245,7 @@ __global__ __launch_bounds__(64) void k_geometry(const float *__restrict__ parti
         hi[0] = fx1; hi[1] = fy1; hi[2] = fz1;
     }
     const float part[3] = {px, py, pz};
-    grid_geometry(lo, hi, part, dims, max_cells, geom);
+    grid_geometry(lo, hi, part, dims, max_cells, order_bits, geom);
 }
 
 __device__ __forceinline__ int cell_axis(float p, float origin, float size, int count) {
@@ -232,18 +254,40 @@ __device__ __forceinline__ int cell_axis(float p, float origin, float size, int 
     return c > count - 1 ? count - 1 : c;
 }
 
+/* Storage-order key of agent i: per axis the cell index c = (int)floorf(q)
+ * clamped, q = (p - origin) / size as in tay/cpu_grid.c:18-41, refined by
+ * s = order_bits sub-cell bits, sub = (int)floorf((q - c) * 2^s) clamped to
+ * [0, 2^s - 1]; the key is the Morton interleave of (c << s) | sub.  With
+ * s = 0 it is the cell's Morton key (tay/cpu_z_grid.c:37-83). */
+__device__ __forceinline__ unsigned storage_key(const unsigned *__restrict__ cols, unsigned cap, unsigned i,
+                                                int dims, const TayGridGeom *__restrict__ g) {
+    const int s = g->order_bits;
+    int idx[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k < dims) {
+            const float q = (__uint_as_float(cols[(2u + k) * cap + i]) - g->origin[k]) / g->cell_size[k];
+            int c = (int)floorf(q);
+            c = c < 0 ? 0 : c;
+            c = c > g->count[k] - 1 ? g->count[k] - 1 : c;
+            int sub = 0;
+            if (s) {
+                sub = (int)floorf((q - (float)c) * (float)(1 << s));
+                sub = sub < 0 ? 0 : sub;
+                sub = sub > (1 << s) - 1 ? (1 << s) - 1 : sub;
+            }
+            idx[k] = (c << s) | sub;
+        }
+    return morton(idx[0], idx[1], idx[2], dims);
+}
+
 __global__ __launch_bounds__(kBuildBlock) void k_keys(const unsigned *__restrict__ cols, unsigned cap, unsigned n,
                                                       int dims, const TayGridGeom *__restrict__ g,
                                                       unsigned *__restrict__ keys) {
     const unsigned i = blockIdx.x * kBuildBlock + threadIdx.x;
     if (i >= n)
         return;
-    int idx[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (k < dims)
-            idx[k] = cell_axis(__uint_as_float(cols[(2u + k) * cap + i]), g->origin[k], g->cell_size[k], g->count[k]);
-    keys[i] = morton(idx[0], idx[1], idx[2], dims);
+    keys[i] = storage_key(cols, cap, i, dims, g);
 }
 
 /* Sort digit of element idx: bits [shift, shift + 8) of its key, or, when
@@ -420,11 +464,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_ranges(const unsigned *__restri
     const unsigned i = blockIdx.x * kBuildBlock + threadIdx.x;
     if (i >= n)
         return;
-    const unsigned key = keys[i];
-    const bool first = i == 0 || keys[i - 1] != key;
-    const bool last = i + 1 == n || keys[i + 1] != key;
+    const unsigned key = cell_of_key(keys[i], dims, g);
+    const bool first = i == 0 || cell_of_key(keys[i - 1], dims, g) != key;
+    const bool last = i + 1 == n || cell_of_key(keys[i + 1], dims, g) != key;
     if (first || last) {
-        TayCellRange *r = ranges + key_to_cell(key, dims, g);
+        TayCellRange *r = ranges + morton_to_cell(key, dims, g);
         if (first)
             r->beg = i;
         if (last)
@@ -446,7 +490,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_permute(const unsigned *__restr
         return;
     const unsigned key = keys[i];
     const unsigned from = vals[i];
-    dst[i] = key;
+    dst[i] = cell_of_key(key, dims, g);
     dst[cap + i] = i - ranges[key_to_cell(key, dims, g)].beg;
     for (unsigned w = 2; w < words; ++w)
         dst[w * cap + i] = src[w * cap + from];

@@ -28,6 +28,10 @@ hipStream_t stream();
 bool device_alloc(void **ptr, size_t bytes, const char *what);
 void device_free(void *ptr);
 
+/* Sub-cell bits per axis of the storage order inside a cell (0: previous
+ * storage order only). */
+constexpr int TAY_GPU_DEFAULT_ORDER_BITS = 2;
+
 constexpr unsigned kSeeBlock = 256;     /* 4 wavefronts, one 64-seer tile each */
 constexpr unsigned kActBlock = 256;
 
@@ -57,6 +61,7 @@ struct TayGpuSpace {
     int is_point;
     int dims;
     float part[4];
+    int order_bits;             /* sub-cell bits of the storage order */
     unsigned long long shared_bytes;
     unsigned long long max_cells;
     unsigned count;

@@ -44,6 +44,7 @@ struct TayGroup {
     unsigned count;
     int box_fixed;
     float4 box_min, box_max;
+    int order_bits;             /* -1: the library's default */
 
     TayGpuSpace *gpu;
     int push_agents;
@@ -188,7 +189,7 @@ int state_compile(TayState *state) {
         float part[4] = {g.min_part_sizes.x, g.min_part_sizes.y, g.min_part_sizes.z, g.min_part_sizes.w};
         g.gpu = tay_gpu_space_create(gpu_kind(state, &g), g.agent_size, g.capacity, g.is_point, g.dims, part,
                                      g.shared_bytes);
-        if (!g.gpu) {
+        if (!g.gpu || (g.order_bits >= 0 && !tay_gpu_space_set_order_bits(g.gpu, g.order_bits))) {
             set_device_error(state, "cannot create the group's device space");
             release_device(state);
             return 0;
@@ -315,6 +316,7 @@ TAY_API TayGroup *tay_add_group(TayState *state, unsigned agent_size, unsigned a
     g->type = TAY_CPU_SIMPLE;
     g->dims = 3;
     g->min_part_sizes.x = g->min_part_sizes.y = g->min_part_sizes.z = g->min_part_sizes.w = 1.0f;
+    g->order_bits = -1;
     state->recompile = 1;
     return g;
 }
@@ -554,6 +556,19 @@ TAY_API int tay_get_pair_count(TayState *state, unsigned pass_index, unsigned lo
     }
     if (!tay_gpu_program_pair_count(state->program, pass_index, pairs)) {
         set_device_error(state, "pair count");
+        return 0;
+    }
+    return 1;
+}
+
+TAY_API int tay_set_order_bits(TayState *state, TayGroup *group, int bits) {
+    if (bits < 0 || bits > 3) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "order bits must be 0 to 3");
+        return 0;
+    }
+    group->order_bits = bits;
+    if (group->gpu && !tay_gpu_space_set_order_bits(group->gpu, bits)) {
+        set_device_error(state, "order bits");
         return 0;
     }
     return 1;

@@ -1,8 +1,9 @@
 """GPU Z-grid parity against the CPU oracle on identical seeded inputs.
 
 The oracle's TAY_GPU_Z_GRID space is the CPU statement of what the GPU grid
-computes (geometry of the reference's OpenCL grid, Morton keys, stable key
-order, x->y->z cell sweep), so with identical inputs every integer output
+computes (geometry of the reference's OpenCL grid, Morton keys, the
+sub-cell storage order inside cells, x->y->z cell sweep), so with identical
+inputs every integer output
 (cell keys, ranks, storage order, neighbour counts) and every float output
 (sums accumulated in the same pair order, positions after the act pass) must
 match bit for bit.  Neighbour counts are also checked against CpuSimple, the
@@ -168,3 +169,19 @@ def test_sorted_storage_properties_large(gpu):
     starts = np.r_[0, np.nonzero(np.diff(keys))[0] + 1]
     expect = np.arange(len(a)) - np.repeat(starts, np.diff(np.r_[starts, len(a)]))
     assert np.array_equal(a["cell_agent_i"], expect)
+
+
+@pytest.mark.parametrize("bits", [0, 1, 3])
+def test_order_bits_bit_exact(gpu, oracle, models, bits):
+    """Every sub-cell storage order (2, the default, is covered above) gives
+    the oracle's storage order and sums bit for bit."""
+    recs = W.flock_agents(20000, seed=63, clusters=8, sigma=30.0, box=600.0)
+    model = W.flock_model(25.0)
+    print("case: flock order_bits=%d" % bits, flush=True)
+    out = []
+    for lib, funcs in ((gpu, None), (oracle, models)):
+        with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, funcs=funcs) as s:
+            assert lib.tay_set_order_bits(s.state, s.group, bits)
+            s.run(4)
+            out.append(s.agents())
+    assert_bitwise(out[0], out[1])

@@ -197,3 +197,56 @@ def test_memory_cap_coarsening(oracle, models):
     assert k > 1
     assert list(count[:3]) == list(-(-base // k)) and cells == np.prod(-(-base // k))
     assert np.array_equal(size[:3].view(np.uint32), (sz / count[:3].astype(np.float32)).view(np.uint32))
+
+
+def _gpu_grid_run(oracle, models, recs, bits, steps=1):
+    """One see pass (no act), so positions are those the sort saw."""
+    m = W.plain_model(40.0)
+    model = W.Model(m.dtype, [m.passes[0]], m.part, m.dims)
+    with W.Simulation(oracle, model, recs, api.TAY_GPU_Z_GRID, funcs=models) as s:
+        assert oracle.tay_set_order_bits(s.state, s.group, bits)
+        s.run(steps)
+        return s.agents(), oracle.pair_counts(s.state, 0), oracle.grid_info(s.state, s.group)
+
+
+def test_storage_order_refines_the_cell_order(oracle, models):
+    """Sub-cell storage order: cells stay contiguous runs in Morton order of
+    their keys (part_i), cell ranks count from 0, agents of a cell are the
+    same set whatever the refinement, and inside a cell the order follows
+    the Morton key of the 4x-finer grid, ties in previous storage order."""
+    recs = W.plain_agents(6000, seed=61)
+    a0, c0, g0 = _gpu_grid_run(oracle, models, recs, 0)
+    a2, c2, g2 = _gpu_grid_run(oracle, models, recs, 2)
+    assert c0 == c2
+    for a in (a0, a2):
+        keys = a["tag"]["part_i"] if "tag" in a.dtype.names else a["part_i"]
+        assert (np.diff(keys.astype(np.int64)) >= 0).all()
+        ranks = a["tag"]["cell_agent_i"] if "tag" in a.dtype.names else a["cell_agent_i"]
+        first = np.r_[True, keys[1:] != keys[:-1]]
+        assert (ranks[first] == 0).all()
+    k0 = a0["part_i"]
+    k2 = a2["part_i"]
+    assert np.array_equal(k0, k2)
+    for key in np.unique(k0)[:200]:
+        s0 = np.sort(a0["result_index"][k0 == key])
+        s2 = np.sort(a2["result_index"][k2 == key])
+        assert np.array_equal(s0, s2)
+    # inside a cell: Morton order of the finer grid (q in float32 as stored)
+    origin, size, counts, _ = g2
+    p = a2["p"][:, :3].astype(np.float32)
+    q = (p - origin[:3]) / size[:3]
+    c = np.clip(np.floor(q).astype(np.int64), 0, counts[:3] - 1)
+    sub = np.clip(np.floor((q - c.astype(np.float32)) * np.float32(4)).astype(np.int64), 0, 3)
+    fine = c * 4 + sub
+    fk = np.array([oracle.oracle_morton_key(int(x), int(y), int(z), 3) for x, y, z in fine], np.int64)
+    same_cell = k2[1:] == k2[:-1]
+    assert (np.diff(fk)[same_cell] >= 0).all()
+    assert (fk >> 6 == k2).all()
+
+
+@pytest.mark.parametrize("bits", [0, 1, 2, 3])
+def test_order_bits_keep_pair_counts(oracle, models, bits):
+    recs = W.plain_agents(3000, seed=62)
+    _, c, _ = _gpu_grid_run(oracle, models, recs, bits)
+    _, c0, _ = _gpu_grid_run(oracle, models, recs, 0)
+    assert c == c0
