@@ -19,7 +19,8 @@
  * cell table without moving the agents.  Threads walk agents in key order
  * for locality.
  *
- * Forward: per step, grid build + one fused see/act kernel; the state at
+ * Forward: per step, grid build + one fused see/act kernel (eight lanes per
+ * agent, their partial sums combined by a butterfly); the state at
  * every checkpoint_every-th step is kept.  Backward: per segment between
  * checkpoints, the forward is recomputed from its checkpoint recording every
  * step's state, sums (S, A, C, W) and neighbour lists (at most K per agent,
@@ -89,8 +90,36 @@ __device__ __forceinline__ PairW pair_weight(float r2, const Prm &c) {
     return o;
 }
 
+/* Lanes per agent in the step and gather kernels: the candidates of an agent
+ * are dealt to LANES lanes (each takes every LANES-th agent of every cell,
+ * cells in x, y, z order) and their partial sums are combined by a butterfly
+ * over the group, which gives every lane the same, run-to-run identical
+ * totals.  Few agents (tens of thousands) would otherwise leave one
+ * wavefront per SIMD and no latency hidden. */
+constexpr unsigned LANES = 8;
+
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (unsigned o = 1; o < LANES; o <<= 1)
+        v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ unsigned group_exclusive_count(unsigned v, unsigned sub) {
+    unsigned inc = v;
+#pragma unroll
+    for (unsigned o = 1; o < LANES; o <<= 1) {
+        const unsigned u = __shfl_up(inc, o, LANES);
+        inc += sub >= o ? u : 0u;
+    }
+    return inc - v;
+}
+
 /* One step for the agents in key order: neighbour sums, acceleration, new
- * state.  RECORD also writes the neighbour list, the sums and the count. */
+ * state.  RECORD also writes the neighbour list (in lane order: lane 0's
+ * neighbours, then lane 1's, ...), the sums and the count; it walks the
+ * candidates twice, counting first so every lane knows where its part of
+ * the list starts. */
 template <bool RECORD>
 __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols, unsigned n,
                                                  const unsigned *__restrict__ ids,
@@ -99,43 +128,66 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
                                                  float *__restrict__ out, unsigned *__restrict__ nbr, unsigned K,
                                                  unsigned *__restrict__ counts, float *__restrict__ sums,
                                                  unsigned *__restrict__ max_count) {
-    const unsigned t = blockIdx.x * 256u + threadIdx.x;
-    unsigned cnt = 0;
-    if (t < n) {
-        const unsigned i = ids[t];
-        const float *p = cols + 2u * n, *v = cols + 5u * n;
-        const float pi[3] = {p[i], p[n + i], p[2u * n + i]};
-        const float vi[3] = {v[i], v[n + i], v[2u * n + i]};
-        int lo[3], hi[3];
-        for (int k = 0; k < 3; ++k)
-            axis_span(pi[k], c.radius, g, k, &lo[k], &hi[k]);
-        float W = 0.0f, A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
-        const unsigned cx = (unsigned)g->count[0], cxy = cx * (unsigned)g->count[1];
+    const unsigned t = (blockIdx.x * 256u + threadIdx.x) / LANES;
+    const unsigned sub = threadIdx.x % LANES;
+    const bool live = t < n;
+    const unsigned i = live ? ids[t] : 0u;
+    const float *p = cols + 2u * n, *v = cols + 5u * n;
+    const float pi[3] = {p[i], p[n + i], p[2u * n + i]};
+    const float vi[3] = {v[i], v[n + i], v[2u * n + i]};
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; ++k)
+        axis_span(pi[k], c.radius, g, k, &lo[k], &hi[k]);
+    const unsigned cx = (unsigned)g->count[0], cxy = cx * (unsigned)g->count[1];
+    /* visits this lane's candidates j of agent i with r2 < R^2 */
+    auto sweep = [&](auto &&visit) {
+        if (!live)
+            return;
         for (int x = lo[0]; x <= hi[0]; ++x)
             for (int y = lo[1]; y <= hi[1]; ++y)
                 for (int z = lo[2]; z <= hi[2]; ++z) {
                     const TayCellRange r = ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
-                    for (unsigned s = r.beg; s < r.end; ++s) {
+                    for (unsigned s = r.beg + sub; s < r.end; s += LANES) {
                         const unsigned j = ids[s];
                         if (j == i)
                             continue;
                         const float d[3] = {p[j] - pi[0], p[n + j] - pi[1], p[2u * n + j] - pi[2]};
                         const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                        if (!(r2 < c.r2max))
-                            continue;
-                        const PairW pw = pair_weight(r2, c);
-                        W += pw.w;
-                        const float vj[3] = {v[j], v[n + j], v[2u * n + j]};
-                        for (int k = 0; k < 3; ++k) {
-                            A[k] += pw.w * (vj[k] - vi[k]);
-                            C[k] += pw.w * d[k];
-                            S[k] -= pw.w * d[k] / pw.q;
-                        }
-                        if (RECORD && cnt < K)
-                            nbr[(size_t)i * K + cnt] = j;
-                        ++cnt;
+                        if (r2 < c.r2max)
+                            visit(j, d, r2);
                     }
                 }
+    };
+    float W = 0.0f, A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
+    unsigned cnt = 0;
+    sweep([&](unsigned j, const float *d, float r2) {
+        const PairW pw = pair_weight(r2, c);
+        W += pw.w;
+        const float vj[3] = {v[j], v[n + j], v[2u * n + j]};
+        for (int k = 0; k < 3; ++k) {
+            A[k] += pw.w * (vj[k] - vi[k]);
+            C[k] += pw.w * d[k];
+            S[k] -= pw.w * d[k] / pw.q;
+        }
+        ++cnt;
+    });
+    W = group_sum(W);
+    for (int k = 0; k < 3; ++k) {
+        A[k] = group_sum(A[k]);
+        C[k] = group_sum(C[k]);
+        S[k] = group_sum(S[k]);
+    }
+    const unsigned before = group_exclusive_count(cnt, sub);
+    const unsigned total = __shfl(before + cnt, LANES - 1u, LANES);
+    if (RECORD) {
+        unsigned e = before;
+        sweep([&](unsigned j, const float *, float) {
+            if (e < K)
+                nbr[(size_t)i * K + e] = j;
+            ++e;
+        });
+    }
+    if (live && sub == 0) {
         const float D = W + 1.0f;
         for (int k = 0; k < 3; ++k) {
             const float a = (c.ws * S[k] + c.wa * A[k] + c.wc * C[k]) / D;
@@ -150,17 +202,18 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
         }
         if (RECORD) {
             sums[(size_t)9 * n + i] = W;
-            counts[i] = cnt;
+            counts[i] = total;
         }
     }
-    const unsigned m = wave_max_u(cnt);
+    const unsigned m = wave_max_u(live ? total : 0u);
     if ((threadIdx.x & 63u) == 0u && m)
         atomicMax(max_count, m);
 }
 
 /* Per agent: the direct adjoint terms of the step, the adjoints of its sums
- * (adj: LS[3], LA[3], LC[3], LW as ten n-long rows) and its share of the
- * weight gradient. */
+ * (adj: LS[3], LA[3], LC[3], LW and two pad words, 48 bytes per agent, so a
+ * neighbour's are three 16-byte loads) and its share of the weight
+ * gradient. */
 __global__ __launch_bounds__(256) void k_df_adj_local(unsigned n, const float *__restrict__ sums, Prm c,
                                                       const float *__restrict__ Lp, const float *__restrict__ Lv,
                                                       float *__restrict__ nLp, float *__restrict__ nLv,
@@ -184,15 +237,20 @@ __global__ __launch_bounds__(256) void k_df_adj_local(unsigned n, const float *_
         dot += La[k] * a;
     }
     double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    float a[12];
     for (int k = 0; k < 3; ++k) {
-        adj[(size_t)k * n + i] = c.ws * La[k] / D;
-        adj[(size_t)(3 + k) * n + i] = c.wa * La[k] / D;
-        adj[(size_t)(6 + k) * n + i] = c.wc * La[k] / D;
+        a[k] = c.ws * La[k] / D;
+        a[3 + k] = c.wa * La[k] / D;
+        a[6 + k] = c.wc * La[k] / D;
         g0 += (double)(La[k] * S[k] / D);
         g1 += (double)(La[k] * A[k] / D);
         g2 += (double)(La[k] * C[k] / D);
     }
-    adj[(size_t)9 * n + i] = -dot / D;
+    a[9] = -dot / D;
+    a[10] = a[11] = 0.0f;
+    float4 *out = (float4 *)(adj + (size_t)12 * i);
+    for (int q = 0; q < 3; ++q)
+        out[q] = make_float4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
     gpart[i] = g0;
     gpart[(size_t)n + i] = g1;
     gpart[(size_t)2 * n + i] = g2;
@@ -224,39 +282,53 @@ __device__ __forceinline__ void pair_gather(const float *d, const PairW &pw, con
     }
 }
 
+__device__ __forceinline__ void load_adj(const float *__restrict__ adj, unsigned i, float *a) {
+    const float4 *q = (const float4 *)(adj + (size_t)12 * i);
+    for (int k = 0; k < 3; ++k) {
+        const float4 v = q[k];
+        a[4 * k] = v.x;
+        a[4 * k + 1] = v.y;
+        a[4 * k + 2] = v.z;
+        a[4 * k + 3] = v.w;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_df_adj_gather(unsigned n, const unsigned *__restrict__ ids,
                                                        const float *__restrict__ cols,
                                                        const unsigned *__restrict__ nbr, unsigned K,
                                                        const unsigned *__restrict__ counts,
                                                        const float *__restrict__ adj, Prm c,
                                                        float *__restrict__ nLp, float *__restrict__ nLv) {
-    const unsigned t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= n)
-        return;
-    const unsigned k = ids[t];
+    const unsigned t = (blockIdx.x * 256u + threadIdx.x) / LANES;
+    const unsigned sub = threadIdx.x % LANES;
+    const bool live = t < n;
+    const unsigned k = live ? ids[t] : 0u;
     const float *p = cols + 2u * n, *v = cols + 5u * n;
     const float pk[3] = {p[k], p[n + k], p[2u * n + k]};
     const float vk[3] = {v[k], v[n + k], v[2u * n + k]};
-    float ak[10];
-    for (int q = 0; q < 10; ++q)
-        ak[q] = adj[(size_t)q * n + k];
+    float ak[12];
+    load_adj(adj, k, ak);
     float gp[3] = {0.0f, 0.0f, 0.0f}, gv[3] = {0.0f, 0.0f, 0.0f};
-    const unsigned cnt = counts[k] < K ? counts[k] : K;
-    for (unsigned e = 0; e < cnt; ++e) {
+    const unsigned cnt = live ? (counts[k] < K ? counts[k] : K) : 0u;
+    for (unsigned e = sub; e < cnt; e += LANES) {
         const unsigned m = nbr[(size_t)k * K + e];
         const float d[3] = {p[m] - pk[0], p[n + m] - pk[1], p[2u * n + m] - pk[2]};
         const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         const PairW pw = pair_weight(r2, c);
         const float dv[3] = {v[m] - vk[0], v[n + m] - vk[1], v[2u * n + m] - vk[2]};
-        float am[10];
-        for (int q = 0; q < 10; ++q)
-            am[q] = adj[(size_t)q * n + m];
+        float am[12];
+        load_adj(adj, m, am);
         pair_gather(d, pw, ak, am, dv, gp, gv);
     }
     for (int q = 0; q < 3; ++q) {
-        nLp[(size_t)q * n + k] += gp[q];
-        nLv[(size_t)q * n + k] += gv[q];
+        gp[q] = group_sum(gp[q]);
+        gv[q] = group_sum(gv[q]);
     }
+    if (live && sub == 0)
+        for (int q = 0; q < 3; ++q) {
+            nLp[(size_t)q * n + k] += gp[q];
+            nLv[(size_t)q * n + k] += gv[q];
+        }
 }
 
 /* One block: gw[r] += sum over agents of gpart[r], in a fixed order. */
@@ -462,7 +534,7 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->seg_counts, 4ull * n * checkpoint_every, "segment counts") &&
               device_alloc((void **)&f->Lp[0], rows3, "adjoint") && device_alloc((void **)&f->Lp[1], rows3, "adjoint") &&
               device_alloc((void **)&f->Lv[0], rows3, "adjoint") && device_alloc((void **)&f->Lv[1], rows3, "adjoint") &&
-              device_alloc((void **)&f->adj, 10ull * n * 4, "sum adjoints") &&
+              device_alloc((void **)&f->adj, 12ull * n * 4, "sum adjoints") &&
               device_alloc((void **)&f->gpart, 3ull * n * 8, "weight gradient parts") &&
               device_alloc((void **)&f->gw, 3 * 8, "weight gradient") &&
               device_alloc((void **)&f->io, rows3, "transfer staging") &&
@@ -480,6 +552,7 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
     if (!s)
         return 0;
     const unsigned n = f->n, blocks = (n + 255u) / 256u;
+    const unsigned lane_blocks = (unsigned)(((unsigned long long)n * LANES + 255u) / 256u);
     for (int k = 0; k < 3; ++k)
         f->weights[k] = weights[k];
     const unsigned segments = (steps + f->every - 1) / f->every;
@@ -510,7 +583,7 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
             ok = ok && hip_ok(hipMemcpyAsync(f->ckpt[t / f->every], f->cols[cur], state_bytes(f),
                                              hipMemcpyDeviceToDevice, s), "checkpoint");
         df_sort(f, f->cols[cur], s);
-        k_df_step<false><<<blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->ranges, f->geom, c, f->cols[cur ^ 1],
+        k_df_step<false><<<lane_blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->ranges, f->geom, c, f->cols[cur ^ 1],
                                                 nullptr, 0, nullptr, nullptr, f->max_count);
         cur ^= 1;
     }
@@ -540,6 +613,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
         return 0;
     }
     const unsigned n = f->n, blocks = (n + 255u) / 256u;
+    const unsigned lane_blocks = (unsigned)(((unsigned long long)n * LANES + 255u) / 256u);
     const size_t rows3 = 3ull * n * sizeof(float);
     unsigned maxc = 0;
     bool ok = hip_ok(hipMemcpyAsync(&maxc, f->max_count, 4, hipMemcpyDeviceToHost, s), "read neighbour maximum") &&
@@ -591,7 +665,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             ok = hip_ok(hipMemcpyAsync(f->seg_ids + (size_t)r * n, f->vals[0], 4ull * n, hipMemcpyDeviceToDevice, s),
                         "record order");
             float *next = t + 1 < t1 ? x + (size_t)kWords * n : f->cols[1];
-            k_df_step<true><<<blocks, 256, 0, s>>>(x, n, f->vals[0], f->ranges, f->geom, c, next,
+            k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], f->ranges, f->geom, c, next,
                                                    f->seg_nbr + (size_t)r * K * n, K, f->seg_counts + (size_t)r * n,
                                                    f->seg_sums + (size_t)r * 10 * n, f->max_count);
         }
@@ -602,7 +676,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             k_df_adj_local<<<blocks, 256, 0, s>>>(n, f->seg_sums + (size_t)r * 10 * n, c, f->Lp[L], f->Lv[L],
                                                   f->Lp[L ^ 1], f->Lv[L ^ 1], f->adj, f->gpart);
             k_df_reduce<<<1, kReduceBlock, 0, s>>>(n, f->gpart, f->gw);
-            k_df_adj_gather<<<blocks, 256, 0, s>>>(n, f->seg_ids + (size_t)r * n, x, f->seg_nbr + (size_t)r * K * n,
+            k_df_adj_gather<<<lane_blocks, 256, 0, s>>>(n, f->seg_ids + (size_t)r * n, x, f->seg_nbr + (size_t)r * K * n,
                                                    K, f->seg_counts + (size_t)r * n, f->adj, c, f->Lp[L ^ 1],
                                                    f->Lv[L ^ 1]);
             L ^= 1;
