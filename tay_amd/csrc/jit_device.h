@@ -233,16 +233,18 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     unsigned cand = 0;      /* lane k: storage index of the chunk's slot k */
 
     /* box-test candidates [c0, c1) of the staged chunk into a 32-bit mask */
+    /* the mask is built from its top bit down (candidate c1 - 1 first):
+     * bits = 2 bits + ok is one add with the compare's carry */
     auto filter = [&](unsigned c0, unsigned c1) {
         unsigned bits = 0;
 #pragma unroll 8
-        for (unsigned c = c0; c < c1; ++c) {
+        for (unsigned c = c1; c-- > c0;) {
             const float *b_lo = chunk_lo[wave][c];
             const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
             bool ok = member & tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
             if (SKIP_SELF)
                 ok = ok & (chunk_idx[wave][c] != i);
-            bits |= ok ? 1u << (c - c0) : 0u;
+            bits = bits + bits + (unsigned)ok;
         }
         return bits;
     };

@@ -185,3 +185,22 @@ def test_order_bits_bit_exact(gpu, oracle, models, bits):
             s.run(4)
             out.append(s.agents())
     assert_bitwise(out[0], out[1])
+
+
+@pytest.mark.parametrize("dims", [1, 2])
+def test_lower_dimensional_grids_bit_exact(gpu, oracle, models, dims):
+    """1D and 2D Z-grids (Morton keys of 16 bits per axis in 2D, the index
+    itself in 1D): the box test uses the first `dims` axes only."""
+    m = W.test_model(30.0)
+    model = W.Model(m.dtype, m.passes, m.part, dims)
+    recs = W.test_agents(6000 if dims == 2 else 3000, seed=70 + dims, clump=0.1)
+    if dims == 1:
+        recs["p"][:, 1:3] = 0.0
+        recs["v"][:, 1:3] = 0.0
+    print("case: test model dims=%d" % dims, flush=True)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, model, recs, 3)
+    assert_bitwise(ga, oa)
+    assert gp == op
+    _, sa, _, sp = run_pair(gpu, oracle, models, model, recs, 1, oracle_space=api.TAY_CPU_SIMPLE)
+    ga1, _, gp1, _ = run_pair(gpu, oracle, models, model, recs, 1)
+    assert gp1 == sp
