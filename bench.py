@@ -117,7 +117,7 @@ def bench_ensemble(lib, args):
     from tay_amd import workloads as W
     ens = W.ensemble(args.ensemble_instances, seed=7)
     radius = [m.passes[0].radius for _, m, _ in ens]
-    factors, widths = (1.0, 1.5, 2.0), (64, 32, 16)
+    factors, widths, walks = (1.0, 1.5, 2.0), (64, 32), (0, 16, 64)
     # tables sized for the smallest cell factor, shared by every run below
     tables = [W.cell_table_mb(side, min(factors) * r) for (_, _, side), r in zip(ens, radius)]
     steps, warm = args.ensemble_steps, 2
@@ -125,16 +125,18 @@ def bench_ensemble(lib, args):
     with W.Batch(lib, ens, radius, tables) as b:
         f = (ctypes.c_float * len(factors))(*factors)
         w = (ctypes.c_uint * len(widths))(*widths)
-        table = (ctypes.c_double * (len(factors) * len(widths)))()
-        bf, bw = ctypes.c_float(), ctypes.c_uint()
-        if not lib.tay_gpu_batch_autotune(b.batch, f, len(factors), w, len(widths), 3, table, ctypes.byref(bf),
-                                          ctypes.byref(bw)):
+        lw = (ctypes.c_uint * len(walks))(*walks)
+        table = (ctypes.c_double * (len(factors) * len(widths) * len(walks)))()
+        bf, bw, bl = ctypes.c_float(), ctypes.c_uint(), ctypes.c_uint()
+        if not lib.tay_gpu_batch_autotune(b.batch, f, len(factors), w, len(widths), lw, len(walks), 3, table,
+                                          ctypes.byref(bf), ctypes.byref(bw), ctypes.byref(bl)):
             raise SystemExit("autotune: " + lib.tay_gpu_last_error().decode())
     part = [bf.value * r for r in radius]
-    width = bw.value
+    width, walk = bw.value, bl.value
 
     with W.Batch(lib, ens, part, tables, count_pairs=True) as twin:
         lib.tay_gpu_batch_set_tile_width(twin.batch, width)
+        lib.tay_gpu_batch_set_lane_walk(twin.batch, walk)
         twin.run(warm)
         lib.tay_gpu_batch_reset_pair_counts(twin.batch)
         twin.run(steps)
@@ -142,6 +144,7 @@ def bench_ensemble(lib, args):
 
     with W.Batch(lib, ens, part, tables) as b:
         lib.tay_gpu_batch_set_tile_width(b.batch, width)
+        lib.tay_gpu_batch_set_lane_walk(b.batch, walk)
         b.run(warm, graph=False)
         lib.tay_gpu_batch_enable_timing(b.batch, 1)
         b.run(steps, graph=False)
@@ -150,6 +153,7 @@ def bench_ensemble(lib, args):
 
     with W.Batch(lib, ens, part, tables) as b:
         lib.tay_gpu_batch_set_tile_width(b.batch, width)
+        lib.tay_gpu_batch_set_lane_walk(b.batch, walk)
         b.run(warm, graph=True)
         lib.tay_gpu_synchronize()
         t0 = time.perf_counter()
@@ -163,6 +167,7 @@ def bench_ensemble(lib, args):
         with W.Simulation(lib, model, rec, api.TAY_GPU_Z_GRID, shared_mb=mb, part=p) as s:
             lib.tay_set_fetch_mode(s.state, api.TAY_FETCH_ON_DEMAND)
             lib.tay_set_tile_width(s.state, width)
+            lib.tay_set_lane_walk(s.state, walk)
             s.run(warm)
             lib.tay_set_pass_timing(s.state, 1)
             lib.tay_gpu_synchronize()
@@ -186,10 +191,10 @@ def bench_ensemble(lib, args):
                          "ms_per_step": inst_wall * 1e3 / steps, "see_ms_per_step": inst_see_ms / steps,
                          "launches_per_pass_per_step": len(ens)},
         "identical": bool(identical),
-        "autotune": {"cell_factors": list(factors), "tile_widths": list(widths),
-                     "ms_per_step": [[table[i * len(widths) + j] for j in range(len(widths))]
-                                     for i in range(len(factors))],
-                     "chosen": {"cell_factor": bf.value, "tile_width": width}},
+        "autotune": {"cell_factors": list(factors), "tile_widths": list(widths), "lane_walks": list(walks),
+                     "ms_per_step": [[[table[(i * len(widths) + j) * len(walks) + k] for k in range(len(walks))]
+                                      for j in range(len(widths))] for i in range(len(factors))],
+                     "chosen": {"cell_factor": bf.value, "tile_width": width, "lane_walk": walk}},
     }
     out["speedup_pairs_per_s"] = out["batched"]["pairs_per_s"] / out["per_instance"]["pairs_per_s"]
     out["speedup_steps_per_s"] = out["batched"]["steps_per_s"] / out["per_instance"]["steps_per_s"]

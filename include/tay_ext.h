@@ -45,6 +45,10 @@ int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
  * 64), as tay_gpu_program_set_tile_width; results do not depend on it. */
 int tay_set_tile_width(TayState *state, unsigned width);
 
+/* Lane-walk width of the see kernels (0 to 64), as
+ * tay_gpu_program_set_lane_walk; results do not depend on it. */
+int tay_set_lane_walk(TayState *state, unsigned width);
+
 /* The device space of a group, valid while the simulation runs (NULL before
  * the first tay_run and for groups not on the GPU). */
 TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group);

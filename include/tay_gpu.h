@@ -158,6 +158,14 @@ int tay_gpu_program_run_pass(TayGpuProgram *program, unsigned pass_index);
  * library). */
 int tay_gpu_program_set_tile_width(TayGpuProgram *program, unsigned width);
 
+/* A lane group of at most `width` seers (0 to 64, default 64) whose cell
+ * box is still too wide for one shared sweep walks per lane (every seer its
+ * own cells, candidates one by one, all members at once) instead of
+ * splitting into smaller groups swept one after another; 0 always splits.
+ * Pays where agents are sparse.  Results do not depend on it.  No reference
+ * counterpart. */
+int tay_gpu_program_set_lane_walk(TayGpuProgram *program, unsigned width);
+
 /* Pairs that passed the box test (agent_see calls) since the last reset;
  * needs count_pairs.  The reference's narrow-phase telemetry counter
  * (tay/space.c:57-60). */
@@ -216,6 +224,9 @@ int tay_gpu_batch_set_contexts(TayGpuBatch *batch, unsigned pass_index, const vo
 /* seers per lane group at the start of a see tile: 64, 32, 16 or 8 */
 int tay_gpu_batch_set_tile_width(TayGpuBatch *batch, unsigned width);
 
+/* as tay_gpu_program_set_lane_walk */
+int tay_gpu_batch_set_lane_walk(TayGpuBatch *batch, unsigned width);
+
 /* The order in which the persistent see kernels take tiles: estimated cost,
  * heaviest first (the default), or storage order (neighbouring tiles share
  * seen cells in cache).  Results do not depend on it. */
@@ -248,15 +259,16 @@ int tay_gpu_batch_pair_count(TayGpuBatch *batch, unsigned pass_index, unsigned l
 int tay_gpu_batch_reset_pair_counts(TayGpuBatch *batch);
 
 /* Autotuner: for every cell factor f (cell size = f x the instance's radius
- * of the first see pass, per axis) and tile width, restores the agents it
- * started from, runs `steps` un-captured steps and times them on the device.
- * table_ms (optional, factors_count x widths_count) receives ms per step.
- * Applies the fastest pair, reports it and leaves the agents as they were.
- * The cell size changes the grid, and so the order of each seer's pairs;
- * the tile width changes neither. */
+ * of the first see pass, per axis), tile width and lane-walk width, restores
+ * the agents it started from, runs `steps` un-captured steps and times them
+ * on the device.  table_ms (optional, factors x widths x lane walks, lane
+ * walk fastest) receives ms per step.  Applies the fastest combination,
+ * reports it and leaves the agents as they were.  The cell size changes the
+ * grid, and so the order of each seer's pairs; the widths do not. */
 int tay_gpu_batch_autotune(TayGpuBatch *batch, const float *cell_factors, unsigned factors_count,
-                           const unsigned *widths, unsigned widths_count, unsigned steps, double *table_ms,
-                           float *best_factor, unsigned *best_width);
+                           const unsigned *widths, unsigned widths_count, const unsigned *lane_walks,
+                           unsigned lane_walks_count, unsigned steps, double *table_ms, float *best_factor,
+                           unsigned *best_width, unsigned *best_lane_walk);
 
 #ifdef __cplusplus
 }

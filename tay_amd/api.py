@@ -110,6 +110,7 @@ _GPU = {
     "tay_get_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_get_gpu_space": (_vp, [_vp, _vp]),
     "tay_set_tile_width": (_i, [_vp, _u]),
+    "tay_set_lane_walk": (_i, [_vp, _u]),
     "tay_set_order_bits": (_i, [_vp, _vp, _i]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),
@@ -139,6 +140,7 @@ _GPU = {
     "tay_gpu_program_enable_timing": (_i, [_vp, _i]),
     "tay_gpu_program_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_gpu_program_set_tile_width": (_i, [_vp, _u]),
+    "tay_gpu_program_set_lane_walk": (_i, [_vp, _u]),
     "tay_gpu_batch_create": (_vp, [C.POINTER(C.c_char_p), _u, C.POINTER(TayGpuBatchPass), _u, _u, _i, _u,
                                    C.POINTER(C.c_uint), C.POINTER(C.c_ulonglong), _i, C.c_char_p, _u]),
     "tay_gpu_batch_destroy": (None, [_vp]),
@@ -149,6 +151,7 @@ _GPU = {
     "tay_gpu_batch_set_contexts": (_i, [_vp, _u, _vp]),
     "tay_gpu_batch_set_tile_width": (_i, [_vp, _u]),
     "tay_gpu_batch_set_tile_order": (_i, [_vp, _i]),
+    "tay_gpu_batch_set_lane_walk": (_i, [_vp, _u]),
     "tay_gpu_batch_set_order_bits": (_i, [_vp, _i]),
     "tay_gpu_batch_push": (_i, [_vp, _vp]),
     "tay_gpu_batch_fetch": (_i, [_vp, _vp]),
@@ -163,8 +166,9 @@ _GPU = {
     "tay_dflock_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "tay_dflock_times": (_i, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tay_dflock_max_neighbours": (_i, [_vp, C.POINTER(C.c_uint)]),
-    "tay_gpu_batch_autotune": (_i, [_vp, C.POINTER(C.c_float), _u, C.POINTER(C.c_uint), _u, _u,
-                                    C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint)]),
+    "tay_gpu_batch_autotune": (_i, [_vp, C.POINTER(C.c_float), _u, C.POINTER(C.c_uint), _u, C.POINTER(C.c_uint), _u,
+                                    _u, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint),
+                                    C.POINTER(C.c_uint)]),
 }
 
 # test-only inspection calls of the oracle (oracle/oracle.h)

@@ -42,6 +42,7 @@ struct TaySeeArgs {
     unsigned long long *pair_count;     /* pairs passing the box test, when counting */
     unsigned tiles;                     /* number of 64-seer tiles */
     unsigned width;                     /* seers per lane group at the start of a tile: 64, 32, 16 or 8 */
+    unsigned lane_walk;                 /* sparse groups of at most this many lanes walk per lane */
 };
 
 struct TayActArgs {
@@ -81,6 +82,7 @@ struct TayBatchSeeArgs {
     const unsigned *context;            /* ctx_words per instance */
     unsigned ctx_words;
     unsigned width;
+    unsigned lane_walk;
     unsigned long long *pair_count;
 };
 

@@ -325,6 +325,7 @@ struct TayGpuBatch {
     unsigned width;
     int order_mode;             /* TayGpuTileOrder */
     int order_bits;             /* sub-cell bits of the storage order */
+    unsigned lane_walk;         /* sparse lane groups up to this width walk per lane */
     unsigned radix_passes;
     std::vector<TayBatchInstance> inst_host;
     std::vector<float> radius_host;     /* first see pass's radii, 4 per instance */
@@ -468,6 +469,7 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
             see.context = p.context;
             see.ctx_words = p.ctx_words;
             see.width = b->width;
+            see.lane_walk = b->lane_walk;
             see.pair_count = p.pair_count;
             arg = &see;
             arg_size = sizeof(see);
@@ -602,6 +604,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     b->width = 64;
     b->order_mode = TAY_GPU_TILE_ORDER_COST;
     b->order_bits = TAY_GPU_DEFAULT_ORDER_BITS;
+    b->lane_walk = TAY_GPU_DEFAULT_LANE_WALK;
     b->radix_passes = kRadixPasses + (instances > 256 ? 2u : 1u);
     b->inst_host = inst;
     b->radius_host.assign(4ull * instances, 0.0f);
@@ -794,6 +797,17 @@ TAY_API int tay_gpu_batch_set_tile_order(TayGpuBatch *b, int order) {
     return 1;
 }
 
+TAY_API int tay_gpu_batch_set_lane_walk(TayGpuBatch *b, unsigned width) {
+    if (width > 64) {
+        set_error("lane walk width %u: must be at most 64", width);
+        return 0;
+    }
+    if (width != b->lane_walk)
+        drop_graph(b);
+    b->lane_walk = width;
+    return 1;
+}
+
 TAY_API int tay_gpu_batch_set_order_bits(TayGpuBatch *b, int bits) {
     if (bits < 0 || bits > 3) {
         set_error("order bits %d: must be 0 to 3", bits);
@@ -936,12 +950,18 @@ TAY_API int tay_gpu_batch_reset_pair_counts(TayGpuBatch *b) {
 }
 
 TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, unsigned factors_count,
-                                   const unsigned *widths, unsigned widths_count, unsigned steps, double *table_ms,
-                                   float *best_factor, unsigned *best_width) {
-    if (!factors_count || !widths_count || !steps) {
-        set_error("autotune needs at least one cell factor, one tile width and one step");
+                                   const unsigned *widths, unsigned widths_count, const unsigned *lane_walks,
+                                   unsigned lane_walks_count, unsigned steps, double *table_ms, float *best_factor,
+                                   unsigned *best_width, unsigned *best_lane_walk) {
+    if (!factors_count || !widths_count || !lane_walks_count || !steps) {
+        set_error("autotune needs at least one cell factor, tile width, lane-walk width and step");
         return 0;
     }
+    for (unsigned j = 0; j < lane_walks_count; ++j)
+        if (lane_walks[j] > 64) {
+            set_error("lane walk width %u: must be at most 64", lane_walks[j]);
+            return 0;
+        }
     for (unsigned j = 0; j < widths_count; ++j)
         if (widths[j] != 64 && widths[j] != 32 && widths[j] != 16 && widths[j] != 8) {
             set_error("tile width %u: must be 64, 32, 16 or 8", widths[j]);
@@ -969,29 +989,32 @@ TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, un
     bool ok = hip_ok(hipMemcpyAsync(b->backup, b->cols[start_cur], col_bytes, hipMemcpyDeviceToDevice, s), "backup");
     double best = 1e300;
     float bf = cell_factors[0];
-    unsigned bw = widths[0];
+    unsigned bw = widths[0], bl = lane_walks[0];
     std::vector<float> parts(4ull * b->instances, 1.0f);
     for (unsigned i = 0; ok && i < factors_count; ++i) {
         for (unsigned k = 0; k < b->instances; ++k)
             for (int d = 0; d < b->dims; ++d)
                 parts[4ull * k + d] = cell_factors[i] * b->radius_host[4ull * k + d];
         ok = tay_gpu_batch_set_part_sizes(b, parts.data());
-        for (unsigned j = 0; ok && j < widths_count; ++j) {
-            b->width = widths[j];
-            b->cur = start_cur;
-            double ms = 0.0;
-            ok = hip_ok(hipMemcpyAsync(b->cols[start_cur], b->backup, col_bytes, hipMemcpyDeviceToDevice, s),
-                        "restore") &&
-                 tay_gpu_batch_run(b, steps, 0, &ms);
-            ms /= steps;
-            if (table_ms)
-                table_ms[(size_t)i * widths_count + j] = ms;
-            if (ok && ms < best) {
-                best = ms;
-                bf = cell_factors[i];
-                bw = widths[j];
+        for (unsigned j = 0; ok && j < widths_count; ++j)
+            for (unsigned l = 0; ok && l < lane_walks_count; ++l) {
+                b->width = widths[j];
+                b->lane_walk = lane_walks[l];
+                b->cur = start_cur;
+                double ms = 0.0;
+                ok = hip_ok(hipMemcpyAsync(b->cols[start_cur], b->backup, col_bytes, hipMemcpyDeviceToDevice, s),
+                            "restore") &&
+                     tay_gpu_batch_run(b, steps, 0, &ms);
+                ms /= steps;
+                if (table_ms)
+                    table_ms[((size_t)i * widths_count + j) * lane_walks_count + l] = ms;
+                if (ok && ms < best) {
+                    best = ms;
+                    bf = cell_factors[i];
+                    bw = widths[j];
+                    bl = lane_walks[l];
+                }
             }
-        }
     }
     b->cur = start_cur;
     ok = ok && hip_ok(hipMemcpyAsync(b->cols[start_cur], b->backup, col_bytes, hipMemcpyDeviceToDevice, s), "restore");
@@ -1001,10 +1024,13 @@ TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, un
     ok = ok && tay_gpu_batch_set_part_sizes(b, parts.data()) && tay_gpu_batch_reset_pair_counts(b);
     drop_graph(b);
     b->width = bw;
+    b->lane_walk = bl;
     b->timing = timing;
     if (best_factor)
         *best_factor = bf;
     if (best_width)
         *best_width = bw;
+    if (best_lane_walk)
+        *best_lane_walk = bl;
     return ok;
 }

@@ -32,6 +32,10 @@ void device_free(void *ptr);
  * storage order only). */
 constexpr int TAY_GPU_DEFAULT_ORDER_BITS = 2;
 
+/* Widest lane group of a see tile that walks per lane when its box is too
+ * wide for one shared sweep (0: split down to single lanes instead). */
+constexpr unsigned TAY_GPU_DEFAULT_LANE_WALK = 64;
+
 constexpr unsigned kSeeBlock = 256;     /* 4 wavefronts, one 64-seer tile each */
 constexpr unsigned kActBlock = 256;
 
@@ -107,6 +111,7 @@ struct TayGpuProgram {
     int count_pairs;
     int timing;
     unsigned width;             /* seers per lane group at the start of a see tile */
+    unsigned lane_walk;         /* sparse lane groups up to this width walk per lane */
 };
 
 #endif

@@ -355,6 +355,7 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
     p->count_pairs = count_pairs;
     p->timing = 0;
     p->width = 64;
+    p->lane_walk = TAY_GPU_DEFAULT_LANE_WALK;
     if (!hip_ok(hipModuleLoadData(&p->module, code.data()), "hipModuleLoadData")) {
         tay_gpu_program_destroy(p);
         return nullptr;
@@ -468,6 +469,7 @@ TAY_API int tay_gpu_program_run_pass(TayGpuProgram *p, unsigned pass_index) {
         see.pair_count = pass.pair_count;
         see.tiles = (a->count + 63u) / 64u;
         see.width = p->width;
+        see.lane_walk = p->lane_walk;
         grid = (see.tiles + (kSeeBlock / 64u) - 1u) / (kSeeBlock / 64u);
         params[0] = &see;
     } else {
@@ -508,6 +510,15 @@ TAY_API int tay_gpu_program_set_tile_width(TayGpuProgram *p, unsigned width) {
         return 0;
     }
     p->width = width;
+    return 1;
+}
+
+TAY_API int tay_gpu_program_set_lane_walk(TayGpuProgram *p, unsigned width) {
+    if (width > 64) {
+        set_error("lane walk width %u: must be at most 64", width);
+        return 0;
+    }
+    p->lane_walk = width;
     return 1;
 }
 

@@ -183,11 +183,14 @@ struct TaySeeView {
  * so every seer still meets its pairs in the reference's sweep order.
  *
  * i0 is the tile's first slot; `width` (a power of two up to 64) is the
- * size of the lane groups a tile starts with.  Returns the pairs this lane
- * passed to the model when COUNT is set. */
+ * size of the lane groups a tile starts with; a group of at most lane_walk
+ * lanes whose box is still too wide walks per lane instead of splitting
+ * further (0: always split).  Returns the pairs this lane passed to the
+ * model when COUNT is set. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
           bool COUNT, unsigned CTX_WORDS, class Call>
-__device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned width0, Call call) {
+__device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned width0, unsigned lane_walk,
+                                                Call call) {
     static_assert(sizeof(typename T::seer) <= SEER_WORDS * 4u, "seer record larger than the group's agent size");
     static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
     constexpr bool SEER_BOX = NARROW == TAY_NARROW_NP || NARROW == TAY_NARROW_NN;
@@ -341,18 +344,65 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                 hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                 cells *= (unsigned)(hi[k] - lo[k] + 1);
             }
-            if (cells > limit && width > 1u) {
+            if (cells > limit && width > 1u && width > lane_walk) {
                 width >>= 1;
                 continue;
             }
-            for (int x = lo[0]; x <= hi[0]; ++x)
-                for (int y = lo[1]; y <= hi[1]; ++y)
-                    for (int z = lo[2]; z <= hi[2]; ++z) {
-                        const TayCellRange cr = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
-                        add_range(cr.beg, cr.end);
+            if (cells > limit && width > 1u) {
+                /* A sparse group of at most lane_walk lanes: every member
+                 * walks the cells of its own box, in x, y, z order and each
+                 * cell in storage order, candidates tested one by one, all
+                 * members at once instead of one sub-group after another. */
+                if (member) {
+                    int plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k < DIMS; ++k) {
+                        const float mlo = (fabsf(a_lo[k]) + r[k]) * 0x1p-20f;
+                        const float mhi = (fabsf(a_hi[k]) + r[k]) * 0x1p-20f;
+                        plo[k] = tay_cell_axis((a_lo[k] - r[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
+                        phi[k] = tay_cell_axis((a_hi[k] + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                     }
-            if (fill)
-                process();
+                    for (int x = plo[0]; x <= phi[0]; ++x)
+                        for (int y = plo[1]; y <= phi[1]; ++y)
+                            for (int z = plo[2]; z <= phi[2]; ++z) {
+                                const TayCellRange cr =
+                                    v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                                for (unsigned j = cr.beg; j < cr.end; ++j) {
+                                    if (SKIP_SELF && j == i)
+                                        continue;
+                                    const unsigned off = j * 4u;
+                                    float b_lo[3] = {0.0f, 0.0f, 0.0f}, b_hi[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                                    for (int k = 0; k < DIMS; ++k) {
+                                        b_lo[k] = __uint_as_float(*(const unsigned *)(seen_col[2 + k] + off));
+                                        b_hi[k] = SEEN_BOX ? __uint_as_float(*(const unsigned *)(seen_col[6 + k] + off))
+                                                           : b_lo[k];
+                                    }
+                                    if (!tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r))
+                                        continue;
+                                    union {
+                                        typename T::seen agent;
+                                        unsigned w[SEEN_WORDS];
+                                    } B;
+#pragma unroll
+                                    for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                                        B.w[k] = *(const unsigned *)(seen_col[k] + off);
+                                    if (COUNT)
+                                        ++pairs;
+                                    call(&A.agent, &B.agent, ctx);
+                                }
+                            }
+                }
+            } else {
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                    for (int y = lo[1]; y <= hi[1]; ++y)
+                        for (int z = lo[2]; z <= hi[2]; ++z) {
+                            const TayCellRange cr = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                            add_range(cr.beg, cr.end);
+                        }
+                if (fill)
+                    process();
+            }
             beg += width;
             while (width < width0 && (beg & (2u * width - 1u)) == 0u)
                 width <<= 1;
@@ -402,7 +452,7 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     for (int k = 0; k < 4; ++k)
         v.radii[k] = args.radii[k];
     const unsigned pairs = tay_see_tile<T, SEER_WORDS, SEEN_WORDS, NARROW, SKIP_SELF, SEEN_GRID, DIMS, COUNT,
-                                        CTX_WORDS>(v, tile * 64u, args.width, call);
+                                        CTX_WORDS>(v, tile * 64u, args.width, args.lane_walk, call);
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);
 }
@@ -439,7 +489,7 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
         for (int d = 0; d < 4; ++d)
             v.radii[d] = args.radii[4u * b + d];
         pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS>(
-            v, tile * 64u, args.width, call);
+            v, tile * 64u, args.width, args.lane_walk, call);
     }
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);

@@ -26,6 +26,7 @@ constexpr unsigned kMaxPasses = 32;     /* tay/config.h:9 */
 constexpr unsigned kMaxSources = 16;    /* tay/ocl.h:4 */
 constexpr unsigned kMaxFuncName = 512;  /* tay/state.h:9 */
 constexpr unsigned long long kMB = 1ull << 20;
+constexpr unsigned kLibraryDefault = ~0u;  /* a tuning knob left at the GPU library's default */
 
 }  // namespace
 
@@ -78,6 +79,7 @@ struct TayState {
     TayGpuProgram *program;
     int count_pairs;
     unsigned tile_width;
+    unsigned lane_walk;
     int timing;
     int fetch_mode;
 };
@@ -231,6 +233,8 @@ int state_compile(TayState *state) {
     if (state->timing)
         tay_gpu_program_enable_timing(state->program, 1);
     tay_gpu_program_set_tile_width(state->program, state->tile_width);
+    if (state->lane_walk != kLibraryDefault)
+        tay_gpu_program_set_lane_walk(state->program, state->lane_walk);
     return 1;
 }
 
@@ -252,6 +256,7 @@ TAY_API TayState *tay_create_state() {
     state->program = nullptr;
     state->count_pairs = 0;
     state->tile_width = 64;
+    state->lane_walk = kLibraryDefault;
     state->timing = 0;
     state->fetch_mode = TAY_FETCH_EVERY_RUN;
     return state;
@@ -571,6 +576,17 @@ TAY_API int tay_set_order_bits(TayState *state, TayGroup *group, int bits) {
         set_device_error(state, "order bits");
         return 0;
     }
+    return 1;
+}
+
+TAY_API int tay_set_lane_walk(TayState *state, unsigned width) {
+    if (width > 64) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "lane walk width must be at most 64");
+        return 0;
+    }
+    state->lane_walk = width;
+    if (state->program)
+        tay_gpu_program_set_lane_walk(state->program, width);
     return 1;
 }
 

@@ -80,11 +80,13 @@ def test_batch_graph_replays_match_eager(gpu):
 def test_batch_tile_width_does_not_change_results(gpu):
     ens = small_ensemble(6, seed=6)
     part, tables = parts_and_tables(ens)
-    print("case: batch tile widths 64/32/8", flush=True)
+    print("case: batch tile widths 64/32/8, lane walks 0/8/64, tile orders", flush=True)
     results = []
-    for width in (64, 32, 8):
+    for width, walk, order in ((64, 0, 0), (32, 0, 1), (8, 0, 0), (64, 8, 0), (64, 64, 1), (16, 64, 0)):
         with W.Batch(gpu, ens, part, tables) as b:
             assert gpu.tay_gpu_batch_set_tile_width(b.batch, width)
+            assert gpu.tay_gpu_batch_set_lane_walk(b.batch, walk)
+            assert gpu.tay_gpu_batch_set_tile_order(b.batch, order)
             b.run(3, graph=True)
             results.append(b.agents())
     for other in results[1:]:
@@ -102,11 +104,12 @@ def test_batch_autotune_restores_agents(gpu):
         before = b.agents()
         factors = (C.c_float * 3)(1.0, 1.5, 2.0)
         widths = (C.c_uint * 2)(64, 16)
-        table = (C.c_double * 6)()
-        bf, bw = C.c_float(), C.c_uint()
-        assert gpu.tay_gpu_batch_autotune(b.batch, factors, 3, widths, 2, 2, table, C.byref(bf), C.byref(bw)), \
-            gpu.tay_gpu_last_error().decode()
-        assert bf.value in (1.0, 1.5, 2.0) and bw.value in (64, 16)
+        walks = (C.c_uint * 2)(0, 64)
+        table = (C.c_double * 12)()
+        bf, bw, bl = C.c_float(), C.c_uint(), C.c_uint()
+        assert gpu.tay_gpu_batch_autotune(b.batch, factors, 3, widths, 2, walks, 2, 2, table, C.byref(bf),
+                                          C.byref(bw), C.byref(bl)), gpu.tay_gpu_last_error().decode()
+        assert bf.value in (1.0, 1.5, 2.0) and bw.value in (64, 16) and bl.value in (0, 64)
         assert all(t > 0 for t in table)
         for x, y in zip(before, b.agents()):
             assert_bitwise(x, y)

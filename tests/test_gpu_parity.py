@@ -204,3 +204,21 @@ def test_lower_dimensional_grids_bit_exact(gpu, oracle, models, dims):
     _, sa, _, sp = run_pair(gpu, oracle, models, model, recs, 1, oracle_space=api.TAY_CPU_SIMPLE)
     ga1, _, gp1, _ = run_pair(gpu, oracle, models, model, recs, 1)
     assert gp1 == sp
+
+
+@pytest.mark.parametrize("walk", [8, 64])
+def test_lane_walk_bit_exact(gpu, oracle, models, walk):
+    """Sparse lane groups walking per lane meet every seer's pairs in the
+    same order: bit-exact with the oracle (sparse clusters make many tiles
+    span far-apart seers)."""
+    recs = W.flock_agents(20000, seed=64, clusters=40, sigma=15.0, box=2000.0)
+    model = W.flock_model(25.0)
+    print("case: flock lane_walk=%d" % walk, flush=True)
+    out = []
+    for lib, funcs in ((gpu, None), (oracle, models)):
+        with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, funcs=funcs) as s:
+            if funcs is None:
+                assert lib.tay_set_lane_walk(s.state, walk)
+            s.run(3)
+            out.append(s.agents())
+    assert_bitwise(out[0], out[1])
