@@ -206,11 +206,11 @@ def test_lower_dimensional_grids_bit_exact(gpu, oracle, models, dims):
     assert gp1 == sp
 
 
-@pytest.mark.parametrize("walk", [8, 64])
+@pytest.mark.parametrize("walk", [0, 8, 64])
 def test_lane_walk_bit_exact(gpu, oracle, models, walk):
     """Sparse lane groups walking per lane meet every seer's pairs in the
-    same order: bit-exact with the oracle (sparse clusters make many tiles
-    span far-apart seers)."""
+    same order as groups that keep splitting (walk 0): bit-exact with the
+    oracle (sparse clusters make many tiles span far-apart seers)."""
     recs = W.flock_agents(20000, seed=64, clusters=40, sigma=15.0, box=2000.0)
     model = W.flock_model(25.0)
     print("case: flock lane_walk=%d" % walk, flush=True)
