@@ -148,6 +148,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t tay_column(const unsigned *bas
     return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(cap * 4u), 0x00020000);
 }
 
+/* bits * 2 + !(m > r): the mask of a chunk is built from its top bit down,
+ * one compare into VCC and one add with that carry per candidate. */
+__device__ __forceinline__ unsigned tay_shift_in_ngt(unsigned bits, float m, float r) {
+    asm("v_cmp_nlt_f32_e32 vcc, %2, %1\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+        : "+v"(bits)
+        : "v"(m), "s"(r)
+        : "vcc");
+    return bits;
+}
+
 __device__ __forceinline__ unsigned tay_wave_max_u(unsigned v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -234,20 +244,39 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     bool member = active;   /* lane belongs to the lane group being processed */
     unsigned fill = 0;      /* slots used in the current chunk (uniform) */
     unsigned cand = 0;      /* lane k: storage index of the chunk's slot k */
+    unsigned self_slot = 64u; /* the chunk slot holding this lane's own seer, 64: none */
 
-    /* box-test candidates [c0, c1) of the staged chunk into a 32-bit mask */
-    /* the mask is built from its top bit down (candidate c1 - 1 first):
-     * bits = 2 bits + ok is one add with the compare's carry */
+    /* Point pairs with one radius on every axis: !(|d_k| > r) for all k is
+     * !(max_k |d_k| > r) (the maximum ignores a NaN difference exactly as
+     * the per-axis test passes it), one max3 and one compare per candidate. */
+    bool iso = NARROW == TAY_NARROW_PP;
+#pragma unroll
+    for (int k = 1; k < DIMS; ++k)
+        iso = iso && r[k] == r[0];
+    const float r0 = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(r[0])));
+
+    /* box-test candidates [c0, c1) of the staged chunk into a 32-bit mask,
+     * built from its top bit down (candidate c1 - 1 first) */
     auto filter = [&](unsigned c0, unsigned c1) {
         unsigned bits = 0;
+        if (iso) {
 #pragma unroll 8
-        for (unsigned c = c1; c-- > c0;) {
-            const float *b_lo = chunk_lo[wave][c];
-            const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
-            bool ok = member & tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
-            if (SKIP_SELF)
-                ok = ok & (chunk_idx[wave][c] != i);
-            bits = bits + bits + (unsigned)ok;
+            for (unsigned c = c1; c-- > c0;) {
+                const float *b = chunk_lo[wave][c];
+                float m = fabsf(a_lo[0] - b[0]);
+#pragma unroll
+                for (int k = 1; k < DIMS; ++k)
+                    m = fmaxf(m, fabsf(a_lo[k] - b[k]));
+                bits = tay_shift_in_ngt(bits, m, r0);
+            }
+        } else {
+#pragma unroll 8
+            for (unsigned c = c1; c-- > c0;) {
+                const float *b_lo = chunk_lo[wave][c];
+                const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
+                const bool ok = tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
+                bits = bits + bits + (unsigned)ok;
+            }
         }
         return bits;
     };
@@ -271,6 +300,10 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         const unsigned lo_bits = filter(0u, fill < 32u ? fill : 32u);
         const unsigned hi_bits = fill > 32u ? filter(32u, fill) : 0u;
         unsigned long long mask = ((unsigned long long)hi_bits << 32) | lo_bits;
+        if (SKIP_SELF && self_slot < 64u)
+            mask &= ~(1ull << self_slot);
+        self_slot = 64u;
+        mask = member ? mask : 0ull;
         if (COUNT)
             pairs += (unsigned)__popcll(mask);
 
@@ -304,6 +337,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
             const unsigned take = end - beg < room ? end - beg : room;
             if (lane - fill < take)
                 cand = beg + (lane - fill);
+            if (SKIP_SELF && i - beg < take)
+                self_slot = fill + (i - beg);
             fill += take;
             beg += take;
             if (fill == 64u)

@@ -147,6 +147,7 @@ class Pass:
     context: np.ndarray = None  # one-element structured array, or None
     radius: float = 0.0
     self_see: int = 0
+    radii: tuple = None         # per-axis see radii (x, y, z); None: radius on every axis
 
 
 @dataclass
@@ -248,7 +249,8 @@ class Simulation:
             fn = C.cast(getattr(funcs, ps.func), C.c_void_p) if funcs is not None else None
             if ps.kind == "see":
                 r = ps.radius
-                lib.tay_add_see(self.state, self.group, self.group, fn, ps.func.encode(), float4(r, r, r, r),
+                rad = float4(*ps.radii, r) if ps.radii is not None else float4(r, r, r, r)
+                lib.tay_add_see(self.state, self.group, self.group, fn, ps.func.encode(), rad,
                                 ps.self_see, ptr, size)
             else:
                 lib.tay_add_act(self.state, self.group, fn, ps.func.encode(), ptr, size)

@@ -206,6 +206,22 @@ def test_lower_dimensional_grids_bit_exact(gpu, oracle, models, dims):
     assert gp1 == sp
 
 
+def test_unequal_axis_radii_bit_exact(gpu, oracle, models):
+    """Per-axis see radii that differ take the per-axis box test instead of
+    the single max-of-axes compare used when all radii are equal."""
+    m = W.plain_model(40.0)
+    see = m.passes[0]
+    model = W.Model(m.dtype, [W.Pass("see", see.func, see.context, 40.0, radii=(40.0, 25.0, 60.0)), m.passes[1]],
+                    m.part)
+    recs = W.plain_agents(8000, seed=65)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, model, recs, 3)
+    assert_bitwise(ga, oa)
+    assert gp == op
+    _, _, sp, _ = run_pair(gpu, oracle, models, model, recs, 1)
+    _, _, _, cp = run_pair(gpu, oracle, models, model, recs, 1, oracle_space=api.TAY_CPU_SIMPLE)
+    assert sp == cp
+
+
 @pytest.mark.parametrize("walk", [0, 8, 64])
 def test_lane_walk_bit_exact(gpu, oracle, models, walk):
     """Sparse lane groups walking per lane meet every seer's pairs in the
