@@ -101,6 +101,15 @@ int tay_gpu_space_grid_info(TayGpuSpace *space, float *origin, float *cell_sizes
  * counterpart. */
 int tay_gpu_copy_probe(unsigned long long bytes, double *ms);
 
+/* Self-check of the device square root that model code's sqrt/sqrtf use:
+ * over the float bit patterns [first, first + count) (at most the 2^32 of
+ * them), mismatches[0] counts inputs where it differs bitwise from the
+ * compiler's IEEE float square root, mismatches[1] from the double square
+ * root rounded to float; first_bad[k] is the smallest such input
+ * (0xFFFFFFFF: none).  Synchronises.  No reference counterpart. */
+int tay_gpu_sqrt_selftest(unsigned long long first, unsigned long long count, unsigned long long *mismatches,
+                          unsigned *first_bad);
+
 /* ------------------------------------------------------------ programs */
 
 typedef struct TayGpuPassDesc {

@@ -37,14 +37,14 @@ HOST_FLAGS = [
 
 SOURCES = ["gpu_space.hip", "gpu_batch.hip", "dflock.hip", "jit.cpp", "tay_api.cpp"]
 HOST_ONLY = {"tay_api.cpp"}
-EMBEDDED = {"device_abi_text": "device_abi.h", "jit_device_text": "jit_device.h"}
+EMBEDDED = {"device_abi_text": "device_abi.h", "device_math_text": "device_math.h", "jit_device_text": "jit_device.h"}
 
 
 def _write_embedded(overrides=None, build_dir=BUILD):
-    """Embed device_abi.h and jit_device.h as string constants (overrides
+    """Embed device_abi.h, device_math.h and jit_device.h as string constants (overrides
     maps a header name to another file to embed in its place)."""
     os.makedirs(build_dir, exist_ok=True)
-    out = ['// generated by tay_amd/build.py from csrc/device_abi.h and csrc/jit_device.h',
+    out = ['// generated by tay_amd/build.py from csrc/device_abi.h, csrc/device_math.h and csrc/jit_device.h',
            '#include "internal.h"', 'namespace tay {']
     for name, header in EMBEDDED.items():
         with open((overrides or {}).get(header, os.path.join(CSRC, header))) as f:

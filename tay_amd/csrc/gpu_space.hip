@@ -24,6 +24,7 @@
  * cell indices and keys match the CPU statement bit for bit.
  */
 #include "internal.h"
+#include "device_math.h"
 
 #include <cfloat>
 #include <cmath>
@@ -357,6 +358,77 @@ TAY_API int tay_gpu_copy_probe(unsigned long long bytes, double *ms) {
     if (ms)
         *ms = t;
     return ok;
+}
+
+/* Every float bit pattern x in [first, first + count): tay_sqrt_rn(x)
+ * against the compiler's IEEE float square root and against the double
+ * square root rounded to float (exact for square roots: 53 >= 2*24 + 2
+ * bits), counting bitwise mismatches; the first mismatching input of each
+ * comparison is kept (atomicMin). */
+__global__ __launch_bounds__(kBuildBlock) void k_sqrt_selftest(unsigned long long first, unsigned long long count,
+                                                               unsigned long long *counts, unsigned *first_bad) {
+    unsigned long long bad_f = 0, bad_d = 0;
+    unsigned min_f = 0xFFFFFFFFu, min_d = 0xFFFFFFFFu;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long k = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; k < count; k += stride) {
+        const unsigned bits = (unsigned)(first + k);
+        const float x = __uint_as_float(bits);
+        const unsigned got = __float_as_uint(tay_sqrt_rn(x));
+        const unsigned want_f = __float_as_uint(__builtin_sqrtf(x));
+        const unsigned want_d = __float_as_uint((float)__builtin_sqrt((double)x));
+        if (got != want_f) {
+            ++bad_f;
+            min_f = bits < min_f ? bits : min_f;
+        }
+        if (got != want_d) {
+            ++bad_d;
+            min_d = bits < min_d ? bits : min_d;
+        }
+    }
+    if (bad_f) {
+        atomicAdd(&counts[0], bad_f);
+        atomicMin(&first_bad[0], min_f);
+    }
+    if (bad_d) {
+        atomicAdd(&counts[1], bad_d);
+        atomicMin(&first_bad[1], min_d);
+    }
+}
+
+TAY_API int tay_gpu_sqrt_selftest(unsigned long long first, unsigned long long count,
+                                  unsigned long long *mismatches, unsigned *first_bad) {
+    hipStream_t s = stream();
+    if (!s)
+        return 0;
+    if (first > 0xFFFFFFFFull || count > 0x100000000ull - first) {
+        set_error("sqrt self-test range [%llu, %llu + %llu) leaves the 32-bit patterns", first, first, count);
+        return 0;
+    }
+    void *buf = nullptr;
+    const unsigned init[4] = {0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu};   /* counts[2] as 4 words, first_bad[2] */
+    bool ok = device_alloc(&buf, 32, "sqrt self-test counters") &&
+              hip_ok(hipMemsetAsync(buf, 0, 16, s), "hipMemsetAsync") &&
+              hip_ok(hipMemcpyAsync((char *)buf + 16, init + 2, 8, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    if (ok && count) {
+        k_sqrt_selftest<<<4096, kBuildBlock, 0, s>>>(first, count, (unsigned long long *)buf,
+                                                     (unsigned *)((char *)buf + 16));
+        ok = launch_ok("sqrt self-test");
+    }
+    unsigned long long host[3] = {0, 0, 0};
+    ok = ok && hip_ok(hipMemcpyAsync(host, buf, 24, hipMemcpyDeviceToHost, s), "hipMemcpyAsync") &&
+         hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    device_free(buf);
+    if (!ok)
+        return 0;
+    if (mismatches) {
+        mismatches[0] = host[0];
+        mismatches[1] = host[1];
+    }
+    if (first_bad) {
+        first_bad[0] = (unsigned)(host[2] & 0xFFFFFFFFu);
+        first_bad[1] = (unsigned)(host[2] >> 32);
+    }
+    return 1;
 }
 
 TAY_API int tay_gpu_space_grid_info(TayGpuSpace *sp, float *origin, float *cell_sizes, int *cell_counts,

@@ -53,6 +53,7 @@ bool valid_func_name(const char *name);
 
 /* Text of device_abi.h and jit_device.h, embedded by the build. */
 extern const char *const device_abi_text;
+extern const char *const device_math_text;
 extern const char *const jit_device_text;
 
 }  // namespace tay

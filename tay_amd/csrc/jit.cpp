@@ -59,6 +59,11 @@ typedef enum TayBool { TAY_FALSE, TAY_TRUE } TayBool;
 #define float4_agent_max(t) (*(float4 *)((char *)(t) + sizeof(TayAgentTag) + sizeof(float4)))
 #define float3_agent_max(t) (*(float3 *)((char *)(t) + sizeof(TayAgentTag) + sizeof(float4)))
 #define float2_agent_max(t) (*(float2 *)((char *)(t) + sizeof(TayAgentTag) + sizeof(float4)))
+__device__ __forceinline__ float tay_sqrt(float x) { return tay_sqrt_rn(x); }
+__device__ __forceinline__ double tay_sqrt(double x) { return __builtin_sqrt(x); }
+template <class I> __device__ __forceinline__ double tay_sqrt(I x) { return __builtin_sqrt((double)x); }
+#define sqrt(x) tay_sqrt(x)
+#define sqrtf(x) tay_sqrt_rn((float)(x))
 )TAYPRELUDE";
 
 const char *const kPreludeEnd = R"TAYPRELUDE(
@@ -66,6 +71,8 @@ const char *const kPreludeEnd = R"TAYPRELUDE(
 #undef global
 #undef constant
 #undef kernel
+#undef sqrt
+#undef sqrtf
 )TAYPRELUDE";
 
 const char *const kOptions[] = {
@@ -132,6 +139,7 @@ TayGpuPassShape shape_of(const TayGpuPassDesc &d) {
 std::string compose_head(const char *const *sources, unsigned sources_count) {
     std::string text;
     text += device_abi_text;
+    text += device_math_text;
     text += kPrelude;
     for (unsigned i = 0; i < sources_count; ++i) {
         append(text, "\n#line 1 \"tay_source_%u\"\n", i);

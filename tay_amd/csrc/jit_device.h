@@ -309,9 +309,12 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
 
         const unsigned rounds = tay_wave_max_u((unsigned)__popcll(mask));
         for (unsigned round = 0; round < rounds; ++round) {
-            if (mask != 0ull) {
-                const unsigned c = (unsigned)__builtin_ctzll(mask);
-                mask &= mask - 1ull;
+            /* lowest set bit and its removal computed on every lane (an empty
+             * mask stays empty), so the mask needs no merge after the body */
+            const bool has = mask != 0ull;
+            const unsigned c = (unsigned)__builtin_ctzll(mask) & 63u;
+            mask &= mask - 1ull;
+            if (has) {
                 const unsigned j = chunk_idx[wave][c];
                 union {
                     typename T::seen agent;

@@ -238,3 +238,16 @@ def test_lane_walk_bit_exact(gpu, oracle, models, walk):
             s.run(3)
             out.append(s.agents())
     assert_bitwise(out[0], out[1])
+
+
+def test_device_sqrt_is_ieee_for_every_float(gpu):
+    """The square root model code gets on the device (sqrt/sqrtf of a float)
+    equals the IEEE square root bit for bit on all 2^32 float inputs, both
+    the compiler's float square root and the double one rounded to float."""
+    import ctypes as C
+    bad = (C.c_ulonglong * 2)()
+    first = (C.c_uint * 2)()
+    print("case: sqrt self-test over 2^32 inputs", flush=True)
+    assert gpu.tay_gpu_sqrt_selftest(0, 1 << 32, bad, first), gpu.tay_gpu_last_error().decode()
+    assert bad[0] == 0, "differs from the float square root at %d inputs, first 0x%08x" % (bad[0], first[0])
+    assert bad[1] == 0, "differs from the rounded double square root at %d inputs, first 0x%08x" % (bad[1], first[1])
