@@ -288,7 +288,7 @@ def bench_adjoint(lib, args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--agents", type=int, default=1_000_000)
     ap.add_argument("--cell-size", type=float, default=25.0, help="Z-grid cell size (min_part_sizes)")
@@ -299,10 +299,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--sph-particles", type=int, default=2_000_000)
-    ap.add_argument("--sph-steps", type=int, default=20)
+    ap.add_argument("--sph-steps", type=int, default=500)
     ap.add_argument("--no-sph", action="store_true", help="skip the SPH measurement")
     ap.add_argument("--adjoint-agents", type=int, default=65536)
-    ap.add_argument("--adjoint-iters", type=int, default=60)
+    ap.add_argument("--adjoint-iters", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-flocking measurement")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one plain device-to-device copy of MIB MiB (reads and writes MIB MiB), the "

@@ -33,6 +33,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 using namespace tay;
@@ -694,6 +695,15 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
                  hip_ok(hipMemsetAsync(p.pair_count, 0, 8, s), "hipMemsetAsync") &&
                  hip_ok(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.kernel, kSeeBlock, 0),
                         "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+            /* tuning: TAY_BATCH_SEE_BLOCKS_PER_CU=<n> replaces the occupancy
+             * query's resident blocks per compute unit */
+            if (const char *e = std::getenv("TAY_BATCH_SEE_BLOCKS_PER_CU")) {
+                const int n = std::atoi(e);
+                if (n >= 1 && n <= 32)
+                    per_cu = n;
+            }
+            if (std::getenv("TAY_BATCH_VERBOSE"))
+                std::fprintf(stderr, "batch see pass %u: %d blocks per CU, %d CUs\n", i, per_cu, cus);
             const unsigned want = (unsigned)(per_cu > 0 ? per_cu : 1) * (unsigned)(cus > 0 ? cus : 1);
             const unsigned need = (b->tiles + kSeeBlock / 64u - 1u) / (kSeeBlock / 64u);
             p.grid = want < need ? want : need;

@@ -153,21 +153,28 @@ std::string compose_head(const char *const *sources, unsigned sources_count) {
     return text;
 }
 
-/* tuning: TAY_SEE_WAVES_PER_EU=<n> asks the compiler to fit n wavefronts per
- * SIMD into the see kernels' register budget */
-std::string see_occupancy() {
-    if (const char *w = std::getenv("TAY_SEE_WAVES_PER_EU")) {
-        const int n = std::atoi(w);
-        if (n >= 1 && n <= 8)
-            return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
-    }
+/* Register budget of the see kernels, as wavefronts per SIMD the compiler
+ * must fit.  The plain see kernel asks for 6 (at most 80 VGPRs): its pair
+ * loop is issue-bound and hides latency with more wavefronts, and what does
+ * not fit is set-up state reloaded once per lane group (1M flocking: 6.1 ->
+ * 5.7 ms against the compiler's own 5).  The persistent batch kernel keeps
+ * the compiler's choice: its sparse instances walk per lane, scattered loads
+ * that more resident wavefronts only make miss more.
+ * TAY_SEE_WAVES_PER_EU=<n> (1-8) overrides both, 0 leaves both to the
+ * compiler. */
+std::string see_occupancy(int fallback) {
+    int n = fallback;
+    if (const char *w = std::getenv("TAY_SEE_WAVES_PER_EU"))
+        n = std::atoi(w);
+    if (n >= 1 && n <= 8)
+        return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
     return std::string();
 }
 
 std::string compose(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
                     unsigned shapes_count, int count_pairs) {
     std::string text = compose_head(sources, sources_count);
-    const std::string occupancy = see_occupancy();
+    const std::string occupancy = see_occupancy(6);
     for (unsigned i = 0; i < shapes_count; ++i) {
         const TayGpuPassShape &p = shapes[i];
         if (p.type == TAY_GPU_PASS_SEE) {
@@ -255,7 +262,7 @@ bool tay::compile_program(const std::string &text, std::vector<char> &code, char
 std::string tay::compose_batch(const char *const *sources, unsigned sources_count, const TayGpuBatchPass *passes,
                                unsigned passes_count, unsigned agent_size, int dims, int count_pairs) {
     std::string text = compose_head(sources, sources_count);
-    const std::string occupancy = see_occupancy();
+    const std::string occupancy = see_occupancy(0);
     for (unsigned i = 0; i < passes_count; ++i) {
         const TayGpuBatchPass &p = passes[i];
         if (p.type == TAY_GPU_PASS_SEE) {

@@ -260,7 +260,7 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     auto filter = [&](unsigned c0, unsigned c1) {
         unsigned bits = 0;
         if (iso) {
-#pragma unroll 8
+#pragma unroll 4
             for (unsigned c = c1; c-- > c0;) {
                 const float *b = chunk_lo[wave][c];
                 float m = fabsf(a_lo[0] - b[0]);
@@ -270,7 +270,7 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                 bits = tay_shift_in_ngt(bits, m, r0);
             }
         } else {
-#pragma unroll 8
+#pragma unroll 4
             for (unsigned c = c1; c-- > c0;) {
                 const float *b_lo = chunk_lo[wave][c];
                 const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
@@ -389,9 +389,14 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
             if (cells > limit && width > 1u) {
                 /* A sparse group of at most lane_walk lanes: every member
                  * walks the cells of its own box, in x, y, z order and each
-                 * cell in storage order, candidates tested one by one, all
-                 * members at once instead of one sub-group after another. */
+                 * cell in storage order, all members at once instead of one
+                 * sub-group after another.  Memory latency is paid per batch:
+                 * the ranges of up to WALK consecutive z cells are loaded
+                 * together, then each cell's candidates WALK at a time
+                 * (positions loaded together, box-tested into a mask, the
+                 * model called for the set bits in ascending order). */
                 if (member) {
+                    constexpr unsigned WALK = 2;
                     int plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
 #pragma unroll
                     for (int k = 0; k < DIMS; ++k) {
@@ -402,32 +407,59 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                     }
                     for (int x = plo[0]; x <= phi[0]; ++x)
                         for (int y = plo[1]; y <= phi[1]; ++y)
-                            for (int z = plo[2]; z <= phi[2]; ++z) {
-                                const TayCellRange cr =
-                                    v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
-                                for (unsigned j = cr.beg; j < cr.end; ++j) {
-                                    if (SKIP_SELF && j == i)
-                                        continue;
-                                    const unsigned off = j * 4u;
-                                    float b_lo[3] = {0.0f, 0.0f, 0.0f}, b_hi[3] = {0.0f, 0.0f, 0.0f};
+                            for (int z0 = plo[2]; z0 <= phi[2]; z0 += (int)WALK) {
+                                TayCellRange crs[WALK];
 #pragma unroll
-                                    for (int k = 0; k < DIMS; ++k) {
-                                        b_lo[k] = __uint_as_float(*(const unsigned *)(seen_col[2 + k] + off));
-                                        b_hi[k] = SEEN_BOX ? __uint_as_float(*(const unsigned *)(seen_col[6 + k] + off))
-                                                           : b_lo[k];
+                                for (unsigned q = 0; q < WALK; ++q) {
+                                    const int z = z0 + (int)q <= phi[2] ? z0 + (int)q : phi[2];
+                                    crs[q] = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                                }
+#pragma unroll
+                                for (unsigned q = 0; q < WALK; ++q) {
+                                    if (z0 + (int)q > phi[2])
+                                        break;
+                                    const unsigned end = crs[q].end;
+                                    for (unsigned j0 = crs[q].beg; j0 < end; j0 += WALK) {
+                                        float b_lo[WALK][3], b_hi[WALK][3];
+#pragma unroll
+                                        for (unsigned t = 0; t < WALK; ++t) {
+                                            const unsigned off = (j0 + t < end ? j0 + t : j0) * 4u;
+#pragma unroll
+                                            for (int k = 0; k < 3; ++k) {
+                                                b_lo[t][k] = k < DIMS ? __uint_as_float(
+                                                                            *(const unsigned *)(seen_col[2 + k] + off))
+                                                                      : 0.0f;
+                                                b_hi[t][k] = SEEN_BOX && k < DIMS
+                                                                 ? __uint_as_float(
+                                                                       *(const unsigned *)(seen_col[6 + k] + off))
+                                                                 : b_lo[t][k];
+                                            }
+                                        }
+                                        unsigned m = 0;
+#pragma unroll
+                                        for (unsigned t = 0; t < WALK; ++t) {
+                                            const unsigned j = j0 + t;
+                                            bool ok = j < end && tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo[t], b_hi[t], r);
+                                            if (SKIP_SELF)
+                                                ok = ok && j != i;
+                                            m |= (unsigned)ok << t;
+                                        }
+                                        while (m) {
+                                            const unsigned j = j0 + (unsigned)__builtin_ctz(m);
+                                            m &= m - 1u;
+                                            const unsigned off = j * 4u;
+                                            union {
+                                                typename T::seen agent;
+                                                unsigned w[SEEN_WORDS];
+                                            } B;
+#pragma unroll
+                                            for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                                                B.w[k] = *(const unsigned *)(seen_col[k] + off);
+                                            if (COUNT)
+                                                ++pairs;
+                                            call(&A.agent, &B.agent, ctx);
+                                        }
                                     }
-                                    if (!tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r))
-                                        continue;
-                                    union {
-                                        typename T::seen agent;
-                                        unsigned w[SEEN_WORDS];
-                                    } B;
-#pragma unroll
-                                    for (unsigned k = 0; k < SEEN_WORDS; ++k)
-                                        B.w[k] = *(const unsigned *)(seen_col[k] + off);
-                                    if (COUNT)
-                                        ++pairs;
-                                    call(&A.agent, &B.agent, ctx);
                                 }
                             }
                 }
@@ -451,11 +483,17 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
             process();
     }
 
+    /* Words the model never writes fold to "unchanged" at compile time and
+     * are not stored; every word it may write is stored, changed or not
+     * (the storage is updated in place), so no copy of the loaded words
+     * has to stay in registers through the sweep. */
     if (active) {
 #pragma unroll
-        for (unsigned k = 0; k < SEER_WORDS; ++k)
-            if (A.w[k] != orig[k])
+        for (unsigned k = 0; k < SEER_WORDS; ++k) {
+            const bool unchanged = A.w[k] == orig[k];
+            if (!(__builtin_constant_p(unchanged) && unchanged))
                 v.seer_out[k * v.seer_cap + i] = A.w[k];
+        }
     }
     return pairs;
 }
