@@ -178,8 +178,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_batch_permute(const unsigned *_
     const bool real = i - in.off < in.count;
     dst[i] = real ? cell_of_key(key, dims, geom + b) : key;
     dst[cap + i] = real ? i - ranges[in.ranges_off + key_to_cell(key, dims, geom + b)].beg : 0u;
-    for (unsigned w = 2; w < words; ++w)
-        dst[w * cap + i] = src[w * cap + from];
+    gather_words(src, dst, cap, from, i, 2u, words);
 }
 
 /* Estimated cost of each tile, bucketed by its binary logarithm; the block
@@ -582,6 +581,11 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     }
     if (slots >= 0xFFFFFFFFull - 64u || slots == 0) {
         set_error("batch of %llu slots: needs between 1 and 2^32 - 65", slots);
+        return nullptr;
+    }
+    /* kernels index a column word as word * slots + slot in 32 bits */
+    if (slots * (agent_size / 4u) > 0xFFFFFFFFull) {
+        set_error("batch of %llu slots of %u-byte agents exceeds 2^32 column words", slots, agent_size);
         return nullptr;
     }
     if (!stream())

@@ -158,6 +158,11 @@ TAY_API TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigne
         set_error("grid structure can only contain point agents");
         return nullptr;
     }
+    /* kernels index a column word as word * capacity + slot in 32 bits */
+    if ((unsigned long long)(agent_size / 4) * (capacity ? capacity : 1) > 0xFFFFFFFFull) {
+        set_error("capacity %u of %u-byte agents exceeds 2^32 column words", capacity, agent_size);
+        return nullptr;
+    }
     if (!stream())
         return nullptr;
 

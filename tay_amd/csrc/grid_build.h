@@ -478,6 +478,35 @@ __global__ __launch_bounds__(kBuildBlock) void k_ranges(const unsigned *__restri
 
 /* Gather every column into key order; word 0 becomes the key (part_i) and
  * word 1 the rank inside the cell (cell_agent_i). */
+/* dst column words [w0, words) of slot i from slot `from` of src, eight
+ * loads in flight before their stores (a runtime-length loop would wait on
+ * every load before the next). */
+__device__ __forceinline__ void gather_words(const unsigned *__restrict__ src, unsigned *__restrict__ dst,
+                                             unsigned cap, unsigned from, unsigned i, unsigned w0, unsigned words) {
+    unsigned w = w0;
+    for (; w + 8u <= words; w += 8u) {
+        unsigned t[8];
+#pragma unroll
+        for (unsigned k = 0; k < 8u; ++k)
+            t[k] = src[(w + k) * cap + from];
+#pragma unroll
+        for (unsigned k = 0; k < 8u; ++k)
+            dst[(w + k) * cap + i] = t[k];
+    }
+    if (w + 4u <= words) {
+        unsigned t[4];
+#pragma unroll
+        for (unsigned k = 0; k < 4u; ++k)
+            t[k] = src[(w + k) * cap + from];
+#pragma unroll
+        for (unsigned k = 0; k < 4u; ++k)
+            dst[(w + k) * cap + i] = t[k];
+        w += 4u;
+    }
+    for (; w < words; ++w)
+        dst[w * cap + i] = src[w * cap + from];
+}
+
 __global__ __launch_bounds__(kBuildBlock) void k_permute(const unsigned *__restrict__ keys,
                                                          const unsigned *__restrict__ vals, unsigned n,
                                                          unsigned words, unsigned cap, int dims,
@@ -492,8 +521,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_permute(const unsigned *__restr
     const unsigned from = vals[i];
     dst[i] = cell_of_key(key, dims, g);
     dst[cap + i] = i - ranges[key_to_cell(key, dims, g)].beg;
-    for (unsigned w = 2; w < words; ++w)
-        dst[w * cap + i] = src[w * cap + from];
+    gather_words(src, dst, cap, from, i, 2u, words);
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_aos_to_soa(const unsigned *__restrict__ aos,
