@@ -225,3 +225,13 @@ def test_no_device_is_reported(tay):
     assert "device not found" in tay.error_message(st)
     assert tay.tay_switch_to_ocl(st) == 0
     tay.tay_destroy_state(st)
+
+
+def test_column_index_range_is_checked_at_creation(tay):
+    """Columns are indexed in 32 bits: a space whose capacity x agent words
+    reaches 2^32 is refused before any device memory is touched."""
+    part = (C.c_float * 4)(25.0, 25.0, 25.0, 25.0)
+    words = A.FLOCK_AGENT.itemsize // 4
+    too_many = (1 << 32) // words + 1
+    assert not tay.tay_gpu_space_create(api.TAY_GPU_KIND_Z_GRID, A.FLOCK_AGENT.itemsize, too_many, 1, 3, part, 1 << 20)
+    assert b"exceeds 2^32 column words" in tay.tay_gpu_last_error()
