@@ -53,7 +53,8 @@ using namespace tay;
 namespace {
 
 constexpr unsigned kWords = 8;      /* tag (2, unused), p (3), v (3) */
-constexpr unsigned kReduceBlock = 1024;
+constexpr unsigned kReduceBlock = 256;
+constexpr unsigned kReduceParts = 64;     /* blocks of the first reduction stage */
 
 struct Prm {
     float radius, r2max, inv_r2, h2, dt, damping;
@@ -116,16 +117,18 @@ __device__ __forceinline__ unsigned group_exclusive_count(unsigned v, unsigned s
 }
 
 /* One step for the agents in key order: neighbour sums, acceleration, new
- * state.  RECORD also writes the neighbour list (in lane order: lane 0's
- * neighbours, then lane 1's, ...), the sums and the count; it walks the
- * candidates twice, counting first so every lane knows where its part of
- * the list starts. */
+ * state.  RECORD also writes the sums and, in the same sweep, each lane's
+ * neighbours to its own list of KL entries with its count; entry e of lane
+ * l of agent i is word (i * KL + e) * LANES + l, so the lanes of an agent
+ * that read their e-th entries together read one contiguous run.  max_count[0] takes the largest
+ * neighbour count of an agent, max_count[1] the largest of one lane, which
+ * sizes the lists of the recording recomputation. */
 template <bool RECORD>
 __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols, unsigned n,
                                                  const unsigned *__restrict__ ids,
                                                  const TayCellRange *__restrict__ ranges,
                                                  const TayGridGeom *__restrict__ g, Prm c,
-                                                 float *__restrict__ out, unsigned *__restrict__ nbr, unsigned K,
+                                                 float *__restrict__ out, unsigned *__restrict__ nbr, unsigned KL,
                                                  unsigned *__restrict__ counts, float *__restrict__ sums,
                                                  unsigned *__restrict__ max_count) {
     const unsigned t = (blockIdx.x * 256u + threadIdx.x) / LANES;
@@ -160,7 +163,10 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     };
     float W = 0.0f, A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
     unsigned cnt = 0;
+    unsigned *list = RECORD ? nbr + (size_t)i * KL * LANES + sub : nullptr;
     sweep([&](unsigned j, const float *d, float r2) {
+        if (RECORD && cnt < KL)
+            list[(size_t)cnt * LANES] = j;
         const PairW pw = pair_weight(r2, c);
         W += pw.w;
         const float vj[3] = {v[j], v[n + j], v[2u * n + j]};
@@ -179,14 +185,8 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     }
     const unsigned before = group_exclusive_count(cnt, sub);
     const unsigned total = __shfl(before + cnt, LANES - 1u, LANES);
-    if (RECORD) {
-        unsigned e = before;
-        sweep([&](unsigned j, const float *, float) {
-            if (e < K)
-                nbr[(size_t)i * K + e] = j;
-            ++e;
-        });
-    }
+    if (RECORD && live)
+        counts[(size_t)i * LANES + sub] = cnt;
     if (live && sub == 0) {
         const float D = W + 1.0f;
         for (int k = 0; k < 3; ++k) {
@@ -200,14 +200,15 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
                 sums[(size_t)(6 + k) * n + i] = C[k];
             }
         }
-        if (RECORD) {
+        if (RECORD)
             sums[(size_t)9 * n + i] = W;
-            counts[i] = total;
-        }
     }
     const unsigned m = wave_max_u(live ? total : 0u);
-    if ((threadIdx.x & 63u) == 0u && m)
+    const unsigned ml = wave_max_u(live ? cnt : 0u);
+    if ((threadIdx.x & 63u) == 0u && m) {
         atomicMax(max_count, m);
+        atomicMax(max_count + 1, ml);
+    }
 }
 
 /* Per agent: the direct adjoint terms of the step, the adjoints of its sums
@@ -295,7 +296,7 @@ __device__ __forceinline__ void load_adj(const float *__restrict__ adj, unsigned
 
 __global__ __launch_bounds__(256) void k_df_adj_gather(unsigned n, const unsigned *__restrict__ ids,
                                                        const float *__restrict__ cols,
-                                                       const unsigned *__restrict__ nbr, unsigned K,
+                                                       const unsigned *__restrict__ nbr, unsigned KL,
                                                        const unsigned *__restrict__ counts,
                                                        const float *__restrict__ adj, Prm c,
                                                        float *__restrict__ nLp, float *__restrict__ nLv) {
@@ -309,9 +310,12 @@ __global__ __launch_bounds__(256) void k_df_adj_gather(unsigned n, const unsigne
     float ak[12];
     load_adj(adj, k, ak);
     float gp[3] = {0.0f, 0.0f, 0.0f}, gv[3] = {0.0f, 0.0f, 0.0f};
-    const unsigned cnt = live ? (counts[k] < K ? counts[k] : K) : 0u;
-    for (unsigned e = sub; e < cnt; e += LANES) {
-        const unsigned m = nbr[(size_t)k * K + e];
+    /* lane sub takes the neighbours lane sub of the step recorded */
+    const unsigned lc = live ? counts[(size_t)k * LANES + sub] : 0u;
+    const unsigned cnt = lc < KL ? lc : KL;
+    const unsigned *list = nbr + (size_t)k * KL * LANES + sub;
+    for (unsigned e = 0; e < cnt; ++e) {
+        const unsigned m = list[(size_t)e * LANES];
         const float d[3] = {p[m] - pk[0], p[n + m] - pk[1], p[2u * n + m] - pk[2]};
         const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         const PairW pw = pair_weight(r2, c);
@@ -332,12 +336,16 @@ __global__ __launch_bounds__(256) void k_df_adj_gather(unsigned n, const unsigne
 }
 
 /* One block: gw[r] += sum over agents of gpart[r], in a fixed order. */
+/* Block b: part[r * kReduceParts + b] = sum of gpart[r] over its contiguous
+ * share of the agents, in a fixed order (strided per thread, then a tree). */
 __global__ __launch_bounds__(kReduceBlock) void k_df_reduce(unsigned n, const double *__restrict__ gpart,
-                                                            double *__restrict__ gw) {
+                                                            double *__restrict__ part) {
     __shared__ double s[kReduceBlock];
+    const unsigned per = (n + kReduceParts - 1) / kReduceParts;
+    const unsigned beg = blockIdx.x * per, end = beg + per < n ? beg + per : n;
     for (unsigned r = 0; r < 3; ++r) {
         double acc = 0.0;
-        for (unsigned i = threadIdx.x; i < n; i += kReduceBlock)
+        for (unsigned i = beg + threadIdx.x; i < end; i += kReduceBlock)
             acc += gpart[(size_t)r * n + i];
         s[threadIdx.x] = acc;
         __syncthreads();
@@ -347,8 +355,18 @@ __global__ __launch_bounds__(kReduceBlock) void k_df_reduce(unsigned n, const do
             __syncthreads();
         }
         if (threadIdx.x == 0)
-            gw[r] += s[0];
+            part[r * kReduceParts + blockIdx.x] = s[0];
         __syncthreads();
+    }
+}
+
+/* gw[r] += the kReduceParts block sums of k_df_reduce, in block order. */
+__global__ __launch_bounds__(64) void k_df_reduce_parts(const double *__restrict__ part, double *__restrict__ gw) {
+    if (threadIdx.x < 3) {
+        double acc = 0.0;
+        for (unsigned b = 0; b < kReduceParts; ++b)
+            acc += part[threadIdx.x * kReduceParts + b];
+        gw[threadIdx.x] += acc;
     }
 }
 
@@ -378,7 +396,7 @@ struct TayDflock {
     unsigned long long max_cells;
     float weights[3];
     unsigned steps;             /* of the last forward */
-    unsigned K;                 /* neighbour-list capacity of the recorded segment */
+    unsigned KL;                /* per-lane neighbour-list capacity of the recorded segment */
 
     float *cols[2];             /* 8 words per agent */
     std::vector<float *> ckpt;  /* 8 words per agent, one per checkpoint */
@@ -393,12 +411,12 @@ struct TayDflock {
     float *seg_states;          /* every x 8 words per agent */
     unsigned *seg_ids;          /* every x n: agents in key order */
     float *seg_sums;            /* every x 10 rows */
-    unsigned *seg_counts;       /* every x n */
-    unsigned *seg_nbr;          /* every x n x K */
+    unsigned *seg_counts;       /* every x n x LANES */
+    unsigned *seg_nbr;          /* every x n x LANES x KL */
     size_t seg_nbr_cap;
     /* adjoint */
     float *Lp[2], *Lv[2], *adj;
-    double *gpart, *gw;
+    double *gpart, *gw, *gparts;
     float *io;                  /* n x 3 host transfer staging */
     double forward_ms, backward_ms;
 };
@@ -479,7 +497,7 @@ TAY_API void tay_dflock_destroy(TayDflock *f) {
     void *bufs[] = {f->cols[0], f->cols[1], f->box_partials, f->geom, f->keys[0], f->keys[1], f->vals[0],
                     f->vals[1], f->block_hist, f->scan_sums, f->ranges, f->max_count, f->seg_states, f->seg_ids,
                     f->seg_sums, f->seg_counts, f->seg_nbr, f->Lp[0], f->Lp[1], f->Lv[0], f->Lv[1], f->adj,
-                    f->gpart, f->gw, f->io};
+                    f->gpart, f->gw, f->gparts, f->io};
     for (void *p : bufs)
         device_free(p);
     delete f;
@@ -527,16 +545,17 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->block_hist, 4ull * kRadix * nblocks, "radix histogram") &&
               device_alloc((void **)&f->scan_sums, 4ull * kScanBlocks, "scan sums") &&
               device_alloc((void **)&f->ranges, sizeof(TayCellRange) * max_cells, "cell table") &&
-              device_alloc((void **)&f->max_count, 4, "neighbour maximum") &&
+              device_alloc((void **)&f->max_count, 8, "neighbour maxima") &&
               device_alloc((void **)&f->seg_states, state_bytes(f) * checkpoint_every, "segment states") &&
               device_alloc((void **)&f->seg_ids, 4ull * n * checkpoint_every, "segment orders") &&
               device_alloc((void **)&f->seg_sums, 10ull * n * 4 * checkpoint_every, "segment sums") &&
-              device_alloc((void **)&f->seg_counts, 4ull * n * checkpoint_every, "segment counts") &&
+              device_alloc((void **)&f->seg_counts, 4ull * LANES * n * checkpoint_every, "segment counts") &&
               device_alloc((void **)&f->Lp[0], rows3, "adjoint") && device_alloc((void **)&f->Lp[1], rows3, "adjoint") &&
               device_alloc((void **)&f->Lv[0], rows3, "adjoint") && device_alloc((void **)&f->Lv[1], rows3, "adjoint") &&
               device_alloc((void **)&f->adj, 12ull * n * 4, "sum adjoints") &&
               device_alloc((void **)&f->gpart, 3ull * n * 8, "weight gradient parts") &&
               device_alloc((void **)&f->gw, 3 * 8, "weight gradient") &&
+              device_alloc((void **)&f->gparts, 3ull * kReduceParts * 8, "weight gradient block sums") &&
               device_alloc((void **)&f->io, rows3, "transfer staging") &&
               hip_ok(hipMemsetAsync(f->geom, 0, sizeof(TayGridGeom), stream()), "hipMemsetAsync");
     if (!ok) {
@@ -569,7 +588,7 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
     ok = ok && hip_ok(hipMemcpyAsync(f->io, v0, rows3, hipMemcpyHostToDevice, s), "upload v0");
     if (ok)
         k_df_unpack<<<blocks, 256, 0, s>>>(f->io, n, f->cols[0] + 5ull * n);
-    ok = ok && hip_ok(hipMemsetAsync(f->max_count, 0, 4, s), "hipMemsetAsync") &&
+    ok = ok && hip_ok(hipMemsetAsync(f->max_count, 0, 8, s), "hipMemsetAsync") &&
          hip_ok(hipStreamSynchronize(s), "upload initial state");
     if (!ok)
         return 0;
@@ -615,13 +634,13 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
     const unsigned n = f->n, blocks = (n + 255u) / 256u;
     const unsigned lane_blocks = (unsigned)(((unsigned long long)n * LANES + 255u) / 256u);
     const size_t rows3 = 3ull * n * sizeof(float);
-    unsigned maxc = 0;
-    bool ok = hip_ok(hipMemcpyAsync(&maxc, f->max_count, 4, hipMemcpyDeviceToHost, s), "read neighbour maximum") &&
-              hip_ok(hipStreamSynchronize(s), "read neighbour maximum");
+    unsigned maxc[2] = {0u, 0u};
+    bool ok = hip_ok(hipMemcpyAsync(maxc, f->max_count, 8, hipMemcpyDeviceToHost, s), "read neighbour maxima") &&
+              hip_ok(hipStreamSynchronize(s), "read neighbour maxima");
     if (!ok)
         return 0;
-    const unsigned K = maxc ? maxc : 1u;
-    const size_t need = (size_t)K * n * f->every;
+    const unsigned KL = maxc[1] ? maxc[1] : 1u;
+    const size_t need = (size_t)KL * LANES * n * f->every;
     if (need > f->seg_nbr_cap) {
         device_free(f->seg_nbr);
         f->seg_nbr = nullptr;
@@ -630,7 +649,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             return 0;
         f->seg_nbr_cap = need;
     }
-    f->K = K;
+    f->KL = KL;
     ok = hip_ok(hipMemcpyAsync(f->io, grad_p_final, rows3, hipMemcpyHostToDevice, s), "upload gradient");
     if (ok)
         k_df_unpack<<<blocks, 256, 0, s>>>(f->io, n, f->Lp[0]);
@@ -642,7 +661,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
         ok = ok && hip_ok(hipMemsetAsync(f->Lv[0], 0, rows3, s), "hipMemsetAsync");
     }
     ok = ok && hip_ok(hipMemsetAsync(f->gw, 0, 24, s), "hipMemsetAsync") &&
-         hip_ok(hipMemsetAsync(f->max_count, 0, 4, s), "hipMemsetAsync") &&
+         hip_ok(hipMemsetAsync(f->max_count, 0, 8, s), "hipMemsetAsync") &&
          hip_ok(hipStreamSynchronize(s), "upload gradient");
     if (!ok)
         return 0;
@@ -666,7 +685,8 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
                         "record order");
             float *next = t + 1 < t1 ? x + (size_t)kWords * n : f->cols[1];
             k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], f->ranges, f->geom, c, next,
-                                                   f->seg_nbr + (size_t)r * K * n, K, f->seg_counts + (size_t)r * n,
+                                                   f->seg_nbr + (size_t)r * KL * LANES * n, KL,
+                                                   f->seg_counts + (size_t)r * LANES * n,
                                                    f->seg_sums + (size_t)r * 10 * n, f->max_count);
         }
         /* reverse the segment's steps */
@@ -675,20 +695,22 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             const float *x = f->seg_states + (size_t)r * kWords * n;
             k_df_adj_local<<<blocks, 256, 0, s>>>(n, f->seg_sums + (size_t)r * 10 * n, c, f->Lp[L], f->Lv[L],
                                                   f->Lp[L ^ 1], f->Lv[L ^ 1], f->adj, f->gpart);
-            k_df_reduce<<<1, kReduceBlock, 0, s>>>(n, f->gpart, f->gw);
-            k_df_adj_gather<<<lane_blocks, 256, 0, s>>>(n, f->seg_ids + (size_t)r * n, x, f->seg_nbr + (size_t)r * K * n,
-                                                   K, f->seg_counts + (size_t)r * n, f->adj, c, f->Lp[L ^ 1],
+            k_df_reduce<<<kReduceParts, kReduceBlock, 0, s>>>(n, f->gpart, f->gparts);
+            k_df_reduce_parts<<<1, 64, 0, s>>>(f->gparts, f->gw);
+            k_df_adj_gather<<<lane_blocks, 256, 0, s>>>(n, f->seg_ids + (size_t)r * n, x,
+                                                   f->seg_nbr + (size_t)r * KL * LANES * n, KL,
+                                                   f->seg_counts + (size_t)r * LANES * n, f->adj, c, f->Lp[L ^ 1],
                                                    f->Lv[L ^ 1]);
             L ^= 1;
         }
         ok = ok && launch_ok("differentiable backward");
     }
     ok = ok && timed_end(ev, s, &f->backward_ms);
-    unsigned recount = 0;
-    ok = ok && hip_ok(hipMemcpyAsync(&recount, f->max_count, 4, hipMemcpyDeviceToHost, s), "read neighbour maximum") &&
-         hip_ok(hipStreamSynchronize(s), "read neighbour maximum");
-    if (ok && recount > K) {
-        set_error("recomputed neighbour count %u exceeds the forward's %u", recount, K);
+    unsigned recount[2] = {0u, 0u};
+    ok = ok && hip_ok(hipMemcpyAsync(recount, f->max_count, 8, hipMemcpyDeviceToHost, s), "read neighbour maxima") &&
+         hip_ok(hipStreamSynchronize(s), "read neighbour maxima");
+    if (ok && recount[1] > KL) {
+        set_error("recomputed per-lane neighbour count %u exceeds the forward's %u", recount[1], KL);
         return 0;
     }
     double gw[3] = {0.0, 0.0, 0.0};
