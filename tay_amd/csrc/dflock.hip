@@ -125,7 +125,7 @@ __device__ __forceinline__ unsigned group_exclusive_count(unsigned v, unsigned s
  * sizes the lists of the recording recomputation. */
 template <bool RECORD>
 __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols, unsigned n,
-                                                 const unsigned *__restrict__ ids,
+                                                 const unsigned *__restrict__ ids, const float *__restrict__ srt,
                                                  const TayCellRange *__restrict__ ranges,
                                                  const TayGridGeom *__restrict__ g, Prm c,
                                                  float *__restrict__ out, unsigned *__restrict__ nbr, unsigned KL,
@@ -138,6 +138,9 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     const float *p = cols + 2u * n, *v = cols + 5u * n;
     const float pi[3] = {p[i], p[n + i], p[2u * n + i]};
     const float vi[3] = {v[i], v[n + i], v[2u * n + i]};
+    /* candidates come from the key-ordered copy: consecutive slots of a
+     * cell are consecutive words */
+    const float *ps = srt, *vs = srt + 3u * n;
     int lo[3], hi[3];
     for (int k = 0; k < 3; ++k)
         axis_span(pi[k], c.radius, g, k, &lo[k], &hi[k]);
@@ -151,25 +154,24 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
                 for (int z = lo[2]; z <= hi[2]; ++z) {
                     const TayCellRange r = ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
                     for (unsigned s = r.beg + sub; s < r.end; s += LANES) {
-                        const unsigned j = ids[s];
-                        if (j == i)
+                        if (s == t)
                             continue;
-                        const float d[3] = {p[j] - pi[0], p[n + j] - pi[1], p[2u * n + j] - pi[2]};
+                        const float d[3] = {ps[s] - pi[0], ps[n + s] - pi[1], ps[2u * n + s] - pi[2]};
                         const float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                         if (r2 < c.r2max)
-                            visit(j, d, r2);
+                            visit(s, d, r2);
                     }
                 }
     };
     float W = 0.0f, A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
     unsigned cnt = 0;
     unsigned *list = RECORD ? nbr + (size_t)i * KL * LANES + sub : nullptr;
-    sweep([&](unsigned j, const float *d, float r2) {
+    sweep([&](unsigned s, const float *d, float r2) {
         if (RECORD && cnt < KL)
-            list[(size_t)cnt * LANES] = j;
+            list[(size_t)cnt * LANES] = ids[s];
         const PairW pw = pair_weight(r2, c);
         W += pw.w;
-        const float vj[3] = {v[j], v[n + j], v[2u * n + j]};
+        const float vj[3] = {vs[s], vs[n + s], vs[2u * n + s]};
         for (int k = 0; k < 3; ++k) {
             A[k] += pw.w * (vj[k] - vi[k]);
             C[k] += pw.w * d[k];
@@ -209,6 +211,23 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
         atomicMax(max_count, m);
         atomicMax(max_count + 1, ml);
     }
+}
+
+/* p and v (words 2..7) of the agents in key order: srt[k * n + s] =
+ * cols[(2 + k) * n + ids[s]]. */
+__global__ __launch_bounds__(256) void k_df_sorted_copy(const float *__restrict__ cols, unsigned n,
+                                                        const unsigned *__restrict__ ids, float *__restrict__ srt) {
+    const unsigned s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n)
+        return;
+    const unsigned j = ids[s];
+    float w[6];
+#pragma unroll
+    for (unsigned k = 0; k < 6u; ++k)
+        w[k] = cols[(2u + k) * n + j];
+#pragma unroll
+    for (unsigned k = 0; k < 6u; ++k)
+        srt[k * n + s] = w[k];
 }
 
 /* Per agent: the direct adjoint terms of the step, the adjoints of its sums
@@ -399,6 +418,7 @@ struct TayDflock {
     unsigned KL;                /* per-lane neighbour-list capacity of the recorded segment */
 
     float *cols[2];             /* 8 words per agent */
+    float *srt;                 /* p and v in key order, 6 words per agent */
     std::vector<float *> ckpt;  /* 8 words per agent, one per checkpoint */
     /* grid */
     float *box_partials;
@@ -497,7 +517,7 @@ TAY_API void tay_dflock_destroy(TayDflock *f) {
     void *bufs[] = {f->cols[0], f->cols[1], f->box_partials, f->geom, f->keys[0], f->keys[1], f->vals[0],
                     f->vals[1], f->block_hist, f->scan_sums, f->ranges, f->max_count, f->seg_states, f->seg_ids,
                     f->seg_sums, f->seg_counts, f->seg_nbr, f->Lp[0], f->Lp[1], f->Lv[0], f->Lv[1], f->adj,
-                    f->gpart, f->gw, f->gparts, f->io};
+                    f->gpart, f->gw, f->gparts, f->io, f->srt};
     for (void *p : bufs)
         device_free(p);
     delete f;
@@ -555,6 +575,7 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->adj, 12ull * n * 4, "sum adjoints") &&
               device_alloc((void **)&f->gpart, 3ull * n * 8, "weight gradient parts") &&
               device_alloc((void **)&f->gw, 3 * 8, "weight gradient") &&
+              device_alloc((void **)&f->srt, 6ull * n * 4, "key-ordered positions and velocities") &&
               device_alloc((void **)&f->gparts, 3ull * kReduceParts * 8, "weight gradient block sums") &&
               device_alloc((void **)&f->io, rows3, "transfer staging") &&
               hip_ok(hipMemsetAsync(f->geom, 0, sizeof(TayGridGeom), stream()), "hipMemsetAsync");
@@ -602,8 +623,9 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
             ok = ok && hip_ok(hipMemcpyAsync(f->ckpt[t / f->every], f->cols[cur], state_bytes(f),
                                              hipMemcpyDeviceToDevice, s), "checkpoint");
         df_sort(f, f->cols[cur], s);
-        k_df_step<false><<<lane_blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->ranges, f->geom, c, f->cols[cur ^ 1],
-                                                nullptr, 0, nullptr, nullptr, f->max_count);
+        k_df_sorted_copy<<<blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->srt);
+        k_df_step<false><<<lane_blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->srt, f->ranges, f->geom, c,
+                                                f->cols[cur ^ 1], nullptr, 0, nullptr, nullptr, f->max_count);
         cur ^= 1;
     }
     ok = ok && launch_ok("differentiable forward") && timed_end(ev, s, &f->forward_ms);
@@ -684,7 +706,8 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             ok = hip_ok(hipMemcpyAsync(f->seg_ids + (size_t)r * n, f->vals[0], 4ull * n, hipMemcpyDeviceToDevice, s),
                         "record order");
             float *next = t + 1 < t1 ? x + (size_t)kWords * n : f->cols[1];
-            k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], f->ranges, f->geom, c, next,
+            k_df_sorted_copy<<<blocks, 256, 0, s>>>(x, n, f->vals[0], f->srt);
+            k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], f->srt, f->ranges, f->geom, c, next,
                                                    f->seg_nbr + (size_t)r * KL * LANES * n, KL,
                                                    f->seg_counts + (size_t)r * LANES * n,
                                                    f->seg_sums + (size_t)r * 10 * n, f->max_count);
