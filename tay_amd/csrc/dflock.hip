@@ -118,9 +118,10 @@ __device__ __forceinline__ unsigned group_exclusive_count(unsigned v, unsigned s
 
 /* One step for the agents in key order: neighbour sums, acceleration, new
  * state.  RECORD also writes the sums and, in the same sweep, each lane's
- * neighbours to its own list of KL entries with its count; entry e of lane
- * l of agent i is word (i * KL + e) * LANES + l, so the lanes of an agent
- * that read their e-th entries together read one contiguous run.  max_count[0] takes the largest
+ * neighbours (their key-order slots) to its own list of KL entries with its
+ * count, everything indexed by the agent's key-order slot t; entry e of
+ * lane l is word (t * KL + e) * LANES + l, so the lanes of an agent that
+ * read their e-th entries together read one contiguous run.  max_count[0] takes the largest
  * neighbour count of an agent, max_count[1] the largest of one lane, which
  * sizes the lists of the recording recomputation. */
 template <bool RECORD>
@@ -165,10 +166,10 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     };
     float W = 0.0f, A[3] = {0.0f, 0.0f, 0.0f}, C[3] = {0.0f, 0.0f, 0.0f}, S[3] = {0.0f, 0.0f, 0.0f};
     unsigned cnt = 0;
-    unsigned *list = RECORD ? nbr + (size_t)i * KL * LANES + sub : nullptr;
+    unsigned *list = RECORD ? nbr + (size_t)t * KL * LANES + sub : nullptr;
     sweep([&](unsigned s, const float *d, float r2) {
         if (RECORD && cnt < KL)
-            list[(size_t)cnt * LANES] = ids[s];
+            list[(size_t)cnt * LANES] = s;
         const PairW pw = pair_weight(r2, c);
         W += pw.w;
         const float vj[3] = {vs[s], vs[n + s], vs[2u * n + s]};
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     const unsigned before = group_exclusive_count(cnt, sub);
     const unsigned total = __shfl(before + cnt, LANES - 1u, LANES);
     if (RECORD && live)
-        counts[(size_t)i * LANES + sub] = cnt;
+        counts[(size_t)t * LANES + sub] = cnt;
     if (live && sub == 0) {
         const float D = W + 1.0f;
         for (int k = 0; k < 3; ++k) {
@@ -197,13 +198,13 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
             out[(5u + k) * n + i] = vn;
             out[(2u + k) * n + i] = pi[k] + c.dt * vn;
             if (RECORD) {
-                sums[(size_t)k * n + i] = S[k];
-                sums[(size_t)(3 + k) * n + i] = A[k];
-                sums[(size_t)(6 + k) * n + i] = C[k];
+                sums[(size_t)k * n + t] = S[k];
+                sums[(size_t)(3 + k) * n + t] = A[k];
+                sums[(size_t)(6 + k) * n + t] = C[k];
             }
         }
         if (RECORD)
-            sums[(size_t)9 * n + i] = W;
+            sums[(size_t)9 * n + t] = W;
     }
     const unsigned m = wave_max_u(live ? total : 0u);
     const unsigned ml = wave_max_u(live ? cnt : 0u);
@@ -234,21 +235,23 @@ __global__ __launch_bounds__(256) void k_df_sorted_copy(const float *__restrict_
  * (adj: LS[3], LA[3], LC[3], LW and two pad words, 48 bytes per agent, so a
  * neighbour's are three 16-byte loads) and its share of the weight
  * gradient. */
-__global__ __launch_bounds__(256) void k_df_adj_local(unsigned n, const float *__restrict__ sums, Prm c,
+__global__ __launch_bounds__(256) void k_df_adj_local(unsigned n, const unsigned *__restrict__ ids,
+                                                      const float *__restrict__ sums, Prm c,
                                                       const float *__restrict__ Lp, const float *__restrict__ Lv,
                                                       float *__restrict__ nLp, float *__restrict__ nLv,
                                                       float *__restrict__ adj, double *__restrict__ gpart) {
-    const unsigned i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n)
+    const unsigned t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n)
         return;
-    const float W = sums[(size_t)9 * n + i];
+    const unsigned i = ids[t];
+    const float W = sums[(size_t)9 * n + t];
     const float D = W + 1.0f;
     float La[3], S[3], A[3], C[3];
     float dot = 0.0f;
     for (int k = 0; k < 3; ++k) {
-        S[k] = sums[(size_t)k * n + i];
-        A[k] = sums[(size_t)(3 + k) * n + i];
-        C[k] = sums[(size_t)(6 + k) * n + i];
+        S[k] = sums[(size_t)k * n + t];
+        A[k] = sums[(size_t)(3 + k) * n + t];
+        C[k] = sums[(size_t)(6 + k) * n + t];
         const float a = (c.ws * S[k] + c.wa * A[k] + c.wc * C[k]) / D;
         const float lvt = Lv[(size_t)k * n + i] + c.dt * Lp[(size_t)k * n + i];
         nLp[(size_t)k * n + i] = Lp[(size_t)k * n + i];
@@ -268,12 +271,12 @@ __global__ __launch_bounds__(256) void k_df_adj_local(unsigned n, const float *_
     }
     a[9] = -dot / D;
     a[10] = a[11] = 0.0f;
-    float4 *out = (float4 *)(adj + (size_t)12 * i);
+    float4 *out = (float4 *)(adj + (size_t)12 * t);
     for (int q = 0; q < 3; ++q)
         out[q] = make_float4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
-    gpart[i] = g0;
-    gpart[(size_t)n + i] = g1;
-    gpart[(size_t)2 * n + i] = g2;
+    gpart[t] = g0;
+    gpart[(size_t)n + t] = g1;
+    gpart[(size_t)2 * n + t] = g2;
 }
 
 /* Pair term of agent k's adjoints from one neighbour m (d = p_m - p_k,
@@ -314,7 +317,7 @@ __device__ __forceinline__ void load_adj(const float *__restrict__ adj, unsigned
 }
 
 __global__ __launch_bounds__(256) void k_df_adj_gather(unsigned n, const unsigned *__restrict__ ids,
-                                                       const float *__restrict__ cols,
+                                                       const float *__restrict__ srt,
                                                        const unsigned *__restrict__ nbr, unsigned KL,
                                                        const unsigned *__restrict__ counts,
                                                        const float *__restrict__ adj, Prm c,
@@ -323,16 +326,19 @@ __global__ __launch_bounds__(256) void k_df_adj_gather(unsigned n, const unsigne
     const unsigned sub = threadIdx.x % LANES;
     const bool live = t < n;
     const unsigned k = live ? ids[t] : 0u;
-    const float *p = cols + 2u * n, *v = cols + 5u * n;
-    const float pk[3] = {p[k], p[n + k], p[2u * n + k]};
-    const float vk[3] = {v[k], v[n + k], v[2u * n + k]};
+    const unsigned tk = live ? t : 0u;
+    /* everything of the step in key order: p and v from its key-ordered
+     * copy, sum adjoints and neighbour lists by key-order slot */
+    const float *p = srt, *v = srt + 3u * n;
+    const float pk[3] = {p[tk], p[n + tk], p[2u * n + tk]};
+    const float vk[3] = {v[tk], v[n + tk], v[2u * n + tk]};
     float ak[12];
-    load_adj(adj, k, ak);
+    load_adj(adj, tk, ak);
     float gp[3] = {0.0f, 0.0f, 0.0f}, gv[3] = {0.0f, 0.0f, 0.0f};
     /* lane sub takes the neighbours lane sub of the step recorded */
-    const unsigned lc = live ? counts[(size_t)k * LANES + sub] : 0u;
+    const unsigned lc = live ? counts[(size_t)t * LANES + sub] : 0u;
     const unsigned cnt = lc < KL ? lc : KL;
-    const unsigned *list = nbr + (size_t)k * KL * LANES + sub;
+    const unsigned *list = nbr + (size_t)tk * KL * LANES + sub;
     for (unsigned e = 0; e < cnt; ++e) {
         const unsigned m = list[(size_t)e * LANES];
         const float d[3] = {p[m] - pk[0], p[n + m] - pk[1], p[2u * n + m] - pk[2]};
@@ -430,7 +436,8 @@ struct TayDflock {
     /* recorded segment */
     float *seg_states;          /* every x 8 words per agent */
     unsigned *seg_ids;          /* every x n: agents in key order */
-    float *seg_sums;            /* every x 10 rows */
+    float *seg_sums;            /* every x 10 rows, key order */
+    float *seg_srt;             /* every x 6 rows: p and v in key order */
     unsigned *seg_counts;       /* every x n x LANES */
     unsigned *seg_nbr;          /* every x n x LANES x KL */
     size_t seg_nbr_cap;
@@ -517,7 +524,7 @@ TAY_API void tay_dflock_destroy(TayDflock *f) {
     void *bufs[] = {f->cols[0], f->cols[1], f->box_partials, f->geom, f->keys[0], f->keys[1], f->vals[0],
                     f->vals[1], f->block_hist, f->scan_sums, f->ranges, f->max_count, f->seg_states, f->seg_ids,
                     f->seg_sums, f->seg_counts, f->seg_nbr, f->Lp[0], f->Lp[1], f->Lv[0], f->Lv[1], f->adj,
-                    f->gpart, f->gw, f->gparts, f->io, f->srt};
+                    f->gpart, f->gw, f->gparts, f->io, f->srt, f->seg_srt};
     for (void *p : bufs)
         device_free(p);
     delete f;
@@ -569,6 +576,7 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->seg_states, state_bytes(f) * checkpoint_every, "segment states") &&
               device_alloc((void **)&f->seg_ids, 4ull * n * checkpoint_every, "segment orders") &&
               device_alloc((void **)&f->seg_sums, 10ull * n * 4 * checkpoint_every, "segment sums") &&
+              device_alloc((void **)&f->seg_srt, 6ull * n * 4 * checkpoint_every, "segment key-ordered states") &&
               device_alloc((void **)&f->seg_counts, 4ull * LANES * n * checkpoint_every, "segment counts") &&
               device_alloc((void **)&f->Lp[0], rows3, "adjoint") && device_alloc((void **)&f->Lp[1], rows3, "adjoint") &&
               device_alloc((void **)&f->Lv[0], rows3, "adjoint") && device_alloc((void **)&f->Lv[1], rows3, "adjoint") &&
@@ -706,8 +714,9 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             ok = hip_ok(hipMemcpyAsync(f->seg_ids + (size_t)r * n, f->vals[0], 4ull * n, hipMemcpyDeviceToDevice, s),
                         "record order");
             float *next = t + 1 < t1 ? x + (size_t)kWords * n : f->cols[1];
-            k_df_sorted_copy<<<blocks, 256, 0, s>>>(x, n, f->vals[0], f->srt);
-            k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], f->srt, f->ranges, f->geom, c, next,
+            float *srt = f->seg_srt + (size_t)r * 6 * n;
+            k_df_sorted_copy<<<blocks, 256, 0, s>>>(x, n, f->vals[0], srt);
+            k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], srt, f->ranges, f->geom, c, next,
                                                    f->seg_nbr + (size_t)r * KL * LANES * n, KL,
                                                    f->seg_counts + (size_t)r * LANES * n,
                                                    f->seg_sums + (size_t)r * 10 * n, f->max_count);
@@ -715,12 +724,12 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
         /* reverse the segment's steps */
         for (unsigned t = t1; ok && t-- > t0;) {
             const unsigned r = t - t0;
-            const float *x = f->seg_states + (size_t)r * kWords * n;
-            k_df_adj_local<<<blocks, 256, 0, s>>>(n, f->seg_sums + (size_t)r * 10 * n, c, f->Lp[L], f->Lv[L],
+            const unsigned *ids = f->seg_ids + (size_t)r * n;
+            k_df_adj_local<<<blocks, 256, 0, s>>>(n, ids, f->seg_sums + (size_t)r * 10 * n, c, f->Lp[L], f->Lv[L],
                                                   f->Lp[L ^ 1], f->Lv[L ^ 1], f->adj, f->gpart);
             k_df_reduce<<<kReduceParts, kReduceBlock, 0, s>>>(n, f->gpart, f->gparts);
             k_df_reduce_parts<<<1, 64, 0, s>>>(f->gparts, f->gw);
-            k_df_adj_gather<<<lane_blocks, 256, 0, s>>>(n, f->seg_ids + (size_t)r * n, x,
+            k_df_adj_gather<<<lane_blocks, 256, 0, s>>>(n, ids, f->seg_srt + (size_t)r * 6 * n,
                                                    f->seg_nbr + (size_t)r * KL * LANES * n, KL,
                                                    f->seg_counts + (size_t)r * LANES * n, f->adj, c, f->Lp[L ^ 1],
                                                    f->Lv[L ^ 1]);
