@@ -48,6 +48,17 @@ SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0,
                "bytes": int((47795.9 * 2.000 + 82266.3 * 1.000) * 1024)}
 
 
+def advance(sim, steps, name):
+    """Untimed steps before the warm-up, in runs of 50 with a progress line
+    after each (a profiler can slow them many times over)."""
+    done = 0
+    while done < steps:
+        k = min(50, steps - done)
+        sim.run(k)
+        done += k
+        print("%s: advanced %d of %d steps" % (name, done, steps), file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -290,6 +301,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--advance", type=int, default=0, metavar="K",
+                    help="run K untimed steps of the flocking state before the warm-up (to time or profile a later "
+                         "window of the trajectory; not part of the default line)")
     ap.add_argument("--agents", type=int, default=1_000_000)
     ap.add_argument("--cell-size", type=float, default=25.0, help="Z-grid cell size (min_part_sizes)")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -328,12 +342,14 @@ def main():
     with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as twin:
         lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
         lib.tay_set_pair_counting(twin.state, 1)
+        advance(twin, args.advance, "counting twin")
         twin.run(args.warmup) if args.warmup else None
         twin.run(args.steps)
         p_see = lib.pair_counts(twin.state, 0)[1]
 
     with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as sim:
         lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
+        advance(sim, args.advance, "timed state")
         if args.warmup:
             sim.run(args.warmup)
         lib.tay_set_pass_timing(sim.state, 1)
@@ -368,7 +384,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "flocking-1M", "agents": args.agents, "clusters": 64, "sigma": 30.0,
                    "box": 2000.0, "radius": 25.0, "cell_size": args.cell_size, "weights": [1.5, 1.0, 1.0], "dt": 0.1,
-                   "max_speed": 2.0, "structure": "TAY_GPU_Z_GRID"},
+                   "max_speed": 2.0, "structure": "TAY_GPU_Z_GRID",
+                   **({"advanced_steps": args.advance} if args.advance else {})},
         "steps_per_s": steps_per_s,
         "p_see_per_step": p_see / args.steps,
         "see_ms_per_step": see_ms / see_launches,
