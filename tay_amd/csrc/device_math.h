@@ -10,23 +10,25 @@
  * and s, x > s_up*s means it lies above the midpoint of s and s_up.  Every
  * other input (zero, denormals and tiny normals, whose residuals would lose
  * bits, infinities, negatives, NaN) takes the compiler's full IEEE square
- * root.  The choice is made once per wavefront, so a wavefront takes one
- * path or the other, never both.  The identity with the IEEE result is
- * checked over all 2^32 inputs by tay_gpu_sqrt_selftest. */
+ * root.  The identity with the IEEE result is checked over all 2^32 inputs
+ * by tay_gpu_sqrt_selftest. */
 #ifndef TAY_DEVICE_MATH_H
 #define TAY_DEVICE_MATH_H
 
 __device__ __forceinline__ float tay_sqrt_rn(float x) {
+    /* the corrected hardware root is computed for every input (straight-line
+     * code, so two calls on one value share it after inlining); only a
+     * wavefront with an input outside [2^-96, +inf) branches to the full
+     * sequence, for those lanes */
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
+    r = __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
     const bool fast = x >= 0x1p-96f && x < __builtin_inff();
-    if (__builtin_amdgcn_ballot_w64(!fast) == 0ull) {
-        const float s = __builtin_amdgcn_sqrtf(x);
-        const float sd = __uint_as_float(__float_as_uint(s) - 1u);
-        const float su = __uint_as_float(__float_as_uint(s) + 1u);
-        float r = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
-        r = __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
-        return r;
-    }
-    return __builtin_sqrtf(x);
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(!fast) != 0ull, 0))
+        r = fast ? r : __builtin_sqrtf(x);
+    return r;
 }
 
 #endif
