@@ -583,9 +583,11 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
         set_error("batch of %llu slots: needs between 1 and 2^32 - 65", slots);
         return nullptr;
     }
-    /* kernels index a column word as word * slots + slot in 32 bits */
-    if (slots * (agent_size / 4u) > 0xFFFFFFFFull) {
-        set_error("batch of %llu slots of %u-byte agents exceeds 2^32 column words", slots, agent_size);
+    /* kernels index a column word as word * slots + slot, and a slot's byte
+     * offset in a column, in 32 bits */
+    if (slots * (agent_size / 4u) > 0xFFFFFFFFull || slots > (1ull << 30)) {
+        set_error("batch of %llu slots of %u-byte agents exceeds 2^32 column words or 2^30 slots", slots,
+                  agent_size);
         return nullptr;
     }
     if (!stream())

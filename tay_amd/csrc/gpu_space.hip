@@ -158,9 +158,14 @@ TAY_API TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigne
         set_error("grid structure can only contain point agents");
         return nullptr;
     }
-    /* kernels index a column word as word * capacity + slot in 32 bits */
+    /* kernels index a column word as word * capacity + slot, and a slot's
+     * byte offset in a column, in 32 bits */
     if ((unsigned long long)(agent_size / 4) * (capacity ? capacity : 1) > 0xFFFFFFFFull) {
         set_error("capacity %u of %u-byte agents exceeds 2^32 column words", capacity, agent_size);
+        return nullptr;
+    }
+    if (capacity > (1u << 30)) {
+        set_error("capacity %u exceeds 2^30 agents (32-bit byte offsets in a column)", capacity);
         return nullptr;
     }
     if (!stream())

@@ -207,9 +207,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     constexpr bool SEEN_BOX = NARROW == TAY_NARROW_PN || NARROW == TAY_NARROW_NN;
     constexpr unsigned WAVES = TAY_SEE_BLOCK / 64u;
 
-    __shared__ float chunk_lo[WAVES][64][4];
+    __shared__ __align__(16) float chunk_lo[WAVES][64][4];
     __shared__ float chunk_hi[SEEN_BOX ? WAVES : 1u][64][4];
-    __shared__ unsigned chunk_idx[WAVES][64];
 
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
@@ -293,7 +292,10 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                         k < DIMS ? __uint_as_float(*(const unsigned *)(seen_col[6 + k] + off))
                                  : 0.0f;
             }
-            chunk_idx[wave][lane] = cand;
+            /* the fourth word carries the candidate's byte offset in the
+             * columns, so the compact loop reads position and offset as
+             * one 16-byte record */
+            chunk_lo[wave][lane][3] = __uint_as_float(off);
         }
         tay_wave_sync();
 
@@ -315,18 +317,19 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
             const unsigned c = (unsigned)__builtin_ctzll(mask) & 63u;
             mask &= mask - 1ull;
             if (has) {
-                const unsigned j = chunk_idx[wave][c];
+                const float4 rec = *(const float4 *)chunk_lo[wave][c];
                 union {
                     typename T::seen agent;
                     unsigned w[SEEN_WORDS];
                 } B;
-                const unsigned off = j * 4u;
+                const unsigned off = __float_as_uint(rec.w);
 #pragma unroll
                 for (unsigned k = 0; k < SEEN_WORDS; ++k)
                     B.w[k] = *(const unsigned *)(seen_col[k] + off);
+                const float pos[3] = {rec.x, rec.y, rec.z};
 #pragma unroll
                 for (int k = 0; k < DIMS; ++k)
-                    B.w[2 + k] = __float_as_uint(chunk_lo[wave][c][k]);
+                    B.w[2 + k] = __float_as_uint(pos[k]);
                 call(&A.agent, &B.agent, ctx);
             }
         }
