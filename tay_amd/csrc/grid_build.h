@@ -389,10 +389,18 @@ __global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict_
     }
 }
 
-/* Stable scatter of one tile.  Wave w owns the contiguous quarter
- * [w * 1024, (w + 1) * 1024) of the tile and places it after the same
- * digits of the waves before it; inside a wave, 16 rounds of 64 keys go in
- * order and lanes with equal digits are ranked by lane (ballot match). */
+__device__ __forceinline__ unsigned digit_of(unsigned key, unsigned val, const unsigned *__restrict__ tile_digit,
+                                             unsigned shift) {
+    return ((tile_digit ? tile_digit[val >> 6] : key) >> shift) & (kRadix - 1);
+}
+
+/* Stable scatter of one tile, through LDS.  Wave w owns the contiguous
+ * quarter [w * 1024, (w + 1) * 1024) of the tile; inside a wave, 16 rounds
+ * of 64 keys go in order and lanes with equal digits are ranked by lane
+ * (ballot match).  Every key first goes to its place in the tile sorted by
+ * digit (in LDS); then the block writes the sorted tile out in order, so
+ * the keys of one digit leave as one contiguous run of the output instead
+ * of one scattered word each. */
 __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *__restrict__ keys_in,
                                                                const unsigned *__restrict__ vals_in,
                                                                unsigned *__restrict__ keys_out,
@@ -401,23 +409,41 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
                                                                unsigned nblocks, int identity_vals,
                                                                const unsigned *__restrict__ tile_digit) {
     __shared__ unsigned wave_hist[kRadixWaves][kRadix];
-    __shared__ unsigned base[kRadixWaves][kRadix];
+    __shared__ unsigned lbase[kRadixWaves][kRadix];   /* next tile slot of (wave, digit) */
+    __shared__ unsigned delta[kRadix];                /* output slot minus tile slot of a digit */
+    __shared__ unsigned wave_sum[kRadixWaves];
+    __shared__ unsigned lkeys[kRadixTile], lvals[kRadixTile];
     const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
     for (unsigned k = t; k < kRadixWaves * kRadix; k += kBuildBlock)
         (&wave_hist[0][0])[k] = 0;
     __syncthreads();
-    const unsigned sub = blockIdx.x * kRadixTile + w * (kRadixRounds * 64);
+    const unsigned tile0 = blockIdx.x * kRadixTile;
+    const unsigned sub = tile0 + w * (kRadixRounds * 64);
     for (unsigned r = 0; r < kRadixRounds; ++r) {
         const unsigned idx = sub + r * 64 + lane;
-        if (idx < n)
-            atomicAdd(&wave_hist[w][radix_digit(keys_in, vals_in, tile_digit, idx, shift)], 1u);
+        if (idx < n) {
+            const unsigned val = tile_digit ? (identity_vals ? idx : vals_in[idx]) : 0u;
+            atomicAdd(&wave_hist[w][digit_of(keys_in[idx], val, tile_digit, shift)], 1u);
+        }
     }
     __syncthreads();
     {
-        unsigned run = offsets[t * nblocks + blockIdx.x];
+        /* thread t is digit t: its count in the tile, the tile slots before
+         * it (scan over the digits), then per wave */
+        unsigned c = 0;
+        for (unsigned ww = 0; ww < kRadixWaves; ++ww)
+            c += wave_hist[ww][t];
+        const unsigned inc = wave_inclusive_sum(c, lane);
+        if (lane == 63u)
+            wave_sum[w] = inc;
+        __syncthreads();
+        unsigned start = inc - c;
+        for (unsigned ww = 0; ww < w; ++ww)
+            start += wave_sum[ww];
+        delta[t] = offsets[t * nblocks + blockIdx.x] - start;
         for (unsigned ww = 0; ww < kRadixWaves; ++ww) {
-            base[ww][t] = run;
-            run += wave_hist[ww][t];
+            lbase[ww][t] = start;
+            start += wave_hist[ww][t];
         }
     }
     __syncthreads();
@@ -427,7 +453,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
         const bool valid = idx < n;
         const unsigned key = valid ? keys_in[idx] : 0u;
         const unsigned val = valid ? (identity_vals ? idx : vals_in[idx]) : 0u;
-        const unsigned d = valid ? radix_digit(keys_in, vals_in, tile_digit, idx, shift) : 0u;
+        const unsigned d = valid ? digit_of(key, val, tile_digit, shift) : 0u;
         unsigned long long m = __ballot(valid);
 #pragma unroll
         for (unsigned b = 0; b < kRadixBits; ++b) {
@@ -437,14 +463,22 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
         }
         const unsigned rank = __popcll(m & below);
         if (valid) {
-            const unsigned pos = base[w][d] + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            const unsigned pos = lbase[w][d] + rank;
+            lkeys[pos] = key;
+            lvals[pos] = val;
         }
         __builtin_amdgcn_wave_barrier();
         if (valid && rank == 0)
-            base[w][d] += __popcll(m);
+            lbase[w][d] += __popcll(m);
         __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    const unsigned count = n - tile0 < kRadixTile ? n - tile0 : kRadixTile;
+    for (unsigned p = t; p < count; p += kBuildBlock) {
+        const unsigned key = lkeys[p], val = lvals[p];
+        const unsigned pos = delta[digit_of(key, val, tile_digit, shift)] + p;
+        keys_out[pos] = key;
+        vals_out[pos] = val;
     }
 }
 
