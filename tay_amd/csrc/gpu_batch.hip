@@ -326,7 +326,7 @@ struct TayGpuBatch {
     int order_mode;             /* TayGpuTileOrder */
     int order_bits;             /* sub-cell bits of the storage order */
     unsigned lane_walk;         /* sparse lane groups up to this width walk per lane */
-    unsigned radix_passes;
+    unsigned sort_tiles;        /* radix tiles of the per-instance segmented sort */
     std::vector<TayBatchInstance> inst_host;
     std::vector<float> radius_host;     /* first see pass's radii, 4 per instance */
     unsigned long long total_cells;
@@ -342,6 +342,7 @@ struct TayGpuBatch {
     unsigned *keys[2];
     unsigned *vals[2];
     unsigned *block_hist;
+    TaySortTile *sort_seg;
     unsigned *scan_sums;
     TayCellRange *ranges;
     unsigned *tile_bucket;
@@ -408,23 +409,24 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
     k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom);
     k_batch_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, b->tile_inst, b->inst, b->dims, b->geom,
                                                       b->keys[0]);
-    const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
+    /* every instance sorted in its own slot range: radix tiles never cross
+     * an instance, and their histograms are laid out instance-major, so the
+     * scan places every key inside its instance (no instance digit pass) */
+    const unsigned nblocks = b->sort_tiles;
     const unsigned total = kRadix * nblocks;
     const unsigned segs = total < kScanBlocks * kBuildBlock ? (total + kBuildBlock - 1) / kBuildBlock : kScanBlocks;
-    for (unsigned pass = 0; pass < b->radix_passes; ++pass) {
-        const bool by_key = pass < kRadixPasses;
-        const unsigned shift = by_key ? pass * kRadixBits : (pass - kRadixPasses) * kRadixBits;
-        const unsigned *digit = by_key ? nullptr : b->tile_inst;
+    for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
+        const unsigned shift = pass * kRadixBits;
         const unsigned in = pass & 1u, out = in ^ 1u;
-        k_radix_hist<<<nblocks, kBuildBlock, 0, s>>>(b->keys[in], b->vals[in], digit, n, shift, b->block_hist,
-                                                     nblocks);
+        k_radix_hist<<<nblocks, kBuildBlock, 0, s>>>(b->keys[in], n, shift, b->block_hist, nblocks, b->sort_seg);
         k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums);
         k_scan_top<<<1, 64, 0, s>>>(b->scan_sums, segs);
         k_scan_down<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums);
         k_radix_scatter<<<nblocks, kBuildBlock, 0, s>>>(b->keys[in], b->vals[in], b->keys[out], b->vals[out], n,
-                                                        shift, b->block_hist, nblocks, pass == 0, digit);
+                                                        shift, b->block_hist, nblocks, pass == 0, b->sort_seg);
     }
-    const unsigned f = b->radix_passes & 1u;
+    /* kRadixPasses is even: the sorted keys and source slots are back in buffer 0 */
+    const unsigned f = 0;
     const unsigned long long per_instance = b->max_instance_cells;
     const unsigned clear_x = blocks_for(per_instance, kBuildBlock, 256);
     k_batch_ranges_clear<<<dim3(clear_x, b->instances), kBuildBlock, 0, s>>>(b->ranges, b->inst, b->geom);
@@ -524,7 +526,7 @@ TAY_API void tay_gpu_batch_destroy(TayGpuBatch *b) {
     }
     void *bufs[] = {b->cols[0], b->cols[1], b->inst, b->tile_inst, b->parts, b->boxes, b->geom, b->keys[0],
                     b->keys[1], b->vals[0], b->vals[1], b->block_hist, b->scan_sums, b->ranges, b->tile_bucket,
-                    b->bucket_hist, b->bucket_off, b->order, b->queues, b->staging, b->backup};
+                    b->bucket_hist, b->bucket_off, b->order, b->queues, b->staging, b->backup, b->sort_seg};
     for (void *p : bufs)
         device_free(p);
     if (b->module)
@@ -612,7 +614,6 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     b->order_mode = TAY_GPU_TILE_ORDER_COST;
     b->order_bits = TAY_GPU_DEFAULT_ORDER_BITS;
     b->lane_walk = TAY_GPU_DEFAULT_LANE_WALK;
-    b->radix_passes = kRadixPasses + (instances > 256 ? 2u : 1u);
     b->inst_host = inst;
     b->radius_host.assign(4ull * instances, 0.0f);
     b->total_cells = cells;
@@ -625,6 +626,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     b->geom = nullptr;
     b->keys[0] = b->keys[1] = b->vals[0] = b->vals[1] = nullptr;
     b->block_hist = b->scan_sums = nullptr;
+    b->sort_seg = nullptr;
     b->ranges = nullptr;
     b->tile_bucket = b->bucket_hist = b->bucket_off = b->order = b->queues = nullptr;
     b->staging = b->backup = nullptr;
@@ -634,7 +636,23 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
 
     hipStream_t s = stream();
     const size_t col_bytes = (size_t)b->words * slots * 4;
-    const unsigned nblocks = (unsigned)((slots + kRadixTile - 1) / kRadixTile);
+    std::vector<TaySortTile> seg;
+    unsigned long long hist_words = 0;
+    for (unsigned i = 0; i < instances; ++i) {
+        const unsigned padded = (inst[i].count + 63u) & ~63u;
+        const unsigned nb = (padded + kRadixTile - 1) / kRadixTile;
+        for (unsigned k = 0; k < nb; ++k) {
+            TaySortTile t;
+            t.start = inst[i].off + k * kRadixTile;
+            t.count = padded - k * kRadixTile < kRadixTile ? padded - k * kRadixTile : kRadixTile;
+            t.hist_base = (unsigned)hist_words + k;
+            t.hist_stride = nb;
+            seg.push_back(t);
+        }
+        hist_words += (unsigned long long)kRadix * nb;
+    }
+    b->sort_tiles = (unsigned)seg.size();
+    const unsigned nblocks = b->sort_tiles ? b->sort_tiles : 1u;
     std::vector<unsigned> tile_inst(b->tiles);
     for (unsigned i = 0; i < instances; ++i)
         for (unsigned t = inst[i].off / 64u; t < (inst[i].off + inst[i].count + 63u) / 64u; ++t)
@@ -654,6 +672,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
               device_alloc((void **)&b->vals[0], 4ull * slots, "sort values") &&
               device_alloc((void **)&b->vals[1], 4ull * slots, "sort values") &&
               device_alloc((void **)&b->block_hist, 4ull * kRadix * nblocks, "radix histogram") &&
+              device_alloc((void **)&b->sort_seg, sizeof(TaySortTile) * nblocks, "sort tiles") &&
               device_alloc((void **)&b->scan_sums, 4ull * kScanBlocks, "scan sums") &&
               device_alloc((void **)&b->ranges, sizeof(TayCellRange) * cells, "cell tables") &&
               device_alloc((void **)&b->tile_bucket, 4ull * b->tiles, "tile buckets") &&
@@ -667,6 +686,8 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
                                     hipMemcpyHostToDevice, s), "upload instances") &&
               hip_ok(hipMemcpyAsync(b->tile_inst, tile_inst.data(), 4ull * b->tiles, hipMemcpyHostToDevice, s),
                      "upload tile instances") &&
+              (seg.empty() || hip_ok(hipMemcpyAsync(b->sort_seg, seg.data(), sizeof(TaySortTile) * seg.size(),
+                                                    hipMemcpyHostToDevice, s), "upload sort tiles")) &&
               hip_ok(hipMemcpyAsync(b->parts, parts.data(), 16ull * instances, hipMemcpyHostToDevice, s),
                      "upload part sizes") &&
               hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize") &&

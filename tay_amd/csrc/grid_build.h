@@ -290,33 +290,42 @@ __global__ __launch_bounds__(kBuildBlock) void k_keys(const unsigned *__restrict
     keys[i] = storage_key(cols, cap, i, dims, g);
 }
 
-/* Sort digit of element idx: bits [shift, shift + 8) of its key, or, when
- * tile_digit is given, of tile_digit[v >> 6] for its value v (the source slot),
- * which sorts a batch by instance after its keys. */
-__device__ __forceinline__ unsigned radix_digit(const unsigned *__restrict__ keys, const unsigned *__restrict__ vals,
-                                                const unsigned *__restrict__ tile_digit, unsigned idx, unsigned shift) {
-    const unsigned src = tile_digit ? tile_digit[vals[idx] >> 6] : keys[idx];
-    return (src >> shift) & (kRadix - 1);
+/* One radix tile of a segmented sort: keys [start, start + count) never
+ * leave their segment; the tile's count of digit d goes to histogram word
+ * hist_base + d * hist_stride.  Laid out segment-major (for each segment,
+ * its digits, for each digit its tiles), one exclusive scan of the
+ * histogram then gives every key its destination inside its own segment. */
+struct TaySortTile {
+    unsigned start, count, hist_base, hist_stride;
+};
+
+/* Tile geometry of block b: a segment tile, or tile b of one unsegmented
+ * sort of n keys over nblocks tiles (histogram digit-major:
+ * hist[d * nblocks + b]). */
+__device__ __forceinline__ TaySortTile sort_tile(const TaySortTile *__restrict__ seg, unsigned b, unsigned n,
+                                                 unsigned nblocks) {
+    if (seg)
+        return seg[b];
+    TaySortTile t;
+    t.start = b * kRadixTile;
+    t.count = n - t.start < kRadixTile ? n - t.start : kRadixTile;
+    t.hist_base = b;
+    t.hist_stride = nblocks;
+    return t;
 }
 
-/* Digit histogram of one 4096-key tile per block, stored digit-major:
- * hist[d * nblocks + block]. */
-__global__ __launch_bounds__(kBuildBlock) void k_radix_hist(const unsigned *__restrict__ keys,
-                                                            const unsigned *__restrict__ vals,
-                                                            const unsigned *__restrict__ tile_digit, unsigned n,
+/* Digit histogram of one tile of at most 4096 keys per block. */
+__global__ __launch_bounds__(kBuildBlock) void k_radix_hist(const unsigned *__restrict__ keys, unsigned n,
                                                             unsigned shift, unsigned *__restrict__ hist,
-                                                            unsigned nblocks) {
+                                                            unsigned nblocks, const TaySortTile *__restrict__ seg) {
     __shared__ unsigned h[kRadix];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const unsigned base = blockIdx.x * kRadixTile;
-    for (unsigned r = 0; r < kRadixTile / kBuildBlock; ++r) {
-        const unsigned idx = base + r * kBuildBlock + threadIdx.x;
-        if (idx < n)
-            atomicAdd(&h[radix_digit(keys, vals, tile_digit, idx, shift)], 1u);
-    }
+    const TaySortTile tile = sort_tile(seg, blockIdx.x, n, nblocks);
+    for (unsigned r = threadIdx.x; r < tile.count; r += kBuildBlock)
+        atomicAdd(&h[(keys[tile.start + r] >> shift) & (kRadix - 1)], 1u);
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[tile.hist_base + threadIdx.x * tile.hist_stride] = h[threadIdx.x];
 }
 
 /* Exclusive scan of the digit-major histogram in place, in three steps over
@@ -389,11 +398,6 @@ __global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict_
     }
 }
 
-__device__ __forceinline__ unsigned digit_of(unsigned key, unsigned val, const unsigned *__restrict__ tile_digit,
-                                             unsigned shift) {
-    return ((tile_digit ? tile_digit[val >> 6] : key) >> shift) & (kRadix - 1);
-}
-
 /* Stable scatter of one tile, through LDS.  Wave w owns the contiguous
  * quarter [w * 1024, (w + 1) * 1024) of the tile; inside a wave, 16 rounds
  * of 64 keys go in order and lanes with equal digits are ranked by lane
@@ -407,7 +411,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
                                                                unsigned *__restrict__ vals_out, unsigned n,
                                                                unsigned shift, const unsigned *__restrict__ offsets,
                                                                unsigned nblocks, int identity_vals,
-                                                               const unsigned *__restrict__ tile_digit) {
+                                                               const TaySortTile *__restrict__ seg) {
     __shared__ unsigned wave_hist[kRadixWaves][kRadix];
     __shared__ unsigned lbase[kRadixWaves][kRadix];   /* next tile slot of (wave, digit) */
     __shared__ unsigned delta[kRadix];                /* output slot minus tile slot of a digit */
@@ -417,14 +421,13 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
     for (unsigned k = t; k < kRadixWaves * kRadix; k += kBuildBlock)
         (&wave_hist[0][0])[k] = 0;
     __syncthreads();
-    const unsigned tile0 = blockIdx.x * kRadixTile;
-    const unsigned sub = tile0 + w * (kRadixRounds * 64);
+    const TaySortTile tile = sort_tile(seg, blockIdx.x, n, nblocks);
+    const unsigned end = tile.start + tile.count;
+    const unsigned sub = tile.start + w * (kRadixRounds * 64);
     for (unsigned r = 0; r < kRadixRounds; ++r) {
         const unsigned idx = sub + r * 64 + lane;
-        if (idx < n) {
-            const unsigned val = tile_digit ? (identity_vals ? idx : vals_in[idx]) : 0u;
-            atomicAdd(&wave_hist[w][digit_of(keys_in[idx], val, tile_digit, shift)], 1u);
-        }
+        if (idx < end)
+            atomicAdd(&wave_hist[w][(keys_in[idx] >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
     {
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
         unsigned start = inc - c;
         for (unsigned ww = 0; ww < w; ++ww)
             start += wave_sum[ww];
-        delta[t] = offsets[t * nblocks + blockIdx.x] - start;
+        delta[t] = offsets[tile.hist_base + t * tile.hist_stride] - start;
         for (unsigned ww = 0; ww < kRadixWaves; ++ww) {
             lbase[ww][t] = start;
             start += wave_hist[ww][t];
@@ -450,10 +453,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
     const unsigned long long below = (1ull << lane) - 1ull;
     for (unsigned r = 0; r < kRadixRounds; ++r) {
         const unsigned idx = sub + r * 64 + lane;
-        const bool valid = idx < n;
+        const bool valid = idx < end;
         const unsigned key = valid ? keys_in[idx] : 0u;
         const unsigned val = valid ? (identity_vals ? idx : vals_in[idx]) : 0u;
-        const unsigned d = valid ? digit_of(key, val, tile_digit, shift) : 0u;
+        const unsigned d = (key >> shift) & (kRadix - 1);
         unsigned long long m = __ballot(valid);
 #pragma unroll
         for (unsigned b = 0; b < kRadixBits; ++b) {
@@ -473,10 +476,9 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    const unsigned count = n - tile0 < kRadixTile ? n - tile0 : kRadixTile;
-    for (unsigned p = t; p < count; p += kBuildBlock) {
+    for (unsigned p = t; p < tile.count; p += kBuildBlock) {
         const unsigned key = lkeys[p], val = lvals[p];
-        const unsigned pos = delta[digit_of(key, val, tile_digit, shift)] + p;
+        const unsigned pos = delta[(key >> shift) & (kRadix - 1)] + p;
         keys_out[pos] = key;
         vals_out[pos] = val;
     }
