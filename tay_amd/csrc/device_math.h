@@ -17,17 +17,17 @@
 
 __device__ __forceinline__ float tay_sqrt_rn(float x) {
     /* the corrected hardware root is computed for every input (straight-line
-     * code, so two calls on one value share it after inlining); only a
-     * wavefront with an input outside [2^-96, +inf) branches to the full
-     * sequence, for those lanes */
+     * code, so two calls on one value share it after inlining); lanes with
+     * an input outside [2^-96, +inf) branch to the full sequence, which a
+     * wavefront skips when none has one */
     const float s = __builtin_amdgcn_sqrtf(x);
     const float sd = __uint_as_float(__float_as_uint(s) - 1u);
     const float su = __uint_as_float(__float_as_uint(s) + 1u);
     float r = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
     r = __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
     const bool fast = x >= 0x1p-96f && x < __builtin_inff();
-    if (__builtin_expect(__builtin_amdgcn_ballot_w64(!fast) != 0ull, 0))
-        r = fast ? r : __builtin_sqrtf(x);
+    if (__builtin_expect(!fast, 0))
+        r = __builtin_sqrtf(x);
     return r;
 }
 
