@@ -141,13 +141,6 @@ __device__ __forceinline__ void tay_wave_sync(void) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* Raw buffer resource over one column of cap words: loads through it take a
- * 32-bit byte offset, so a lane's address costs one shift, not a 64-bit add
- * per column. */
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t tay_column(const unsigned *base, unsigned cap) {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)(cap * 4u), 0x00020000);
-}
-
 /* bits * 2 + !(m > r): the mask of a chunk is built from its top bit down,
  * one compare into VCC and one add with that carry per candidate. */
 __device__ __forceinline__ unsigned tay_shift_in_ngt(unsigned bits, float m, float r) {

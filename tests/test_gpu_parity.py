@@ -251,3 +251,16 @@ def test_device_sqrt_is_ieee_for_every_float(gpu):
     assert gpu.tay_gpu_sqrt_selftest(0, 1 << 32, bad, first), gpu.tay_gpu_last_error().decode()
     assert bad[0] == 0, "differs from the float square root at %d inputs, first 0x%08x" % (bad[0], first[0])
     assert bad[1] == 0, "differs from the rounded double square root at %d inputs, first 0x%08x" % (bad[1], first[1])
+
+
+def test_flocking_coincident_agents_bit_exact(gpu, oracle, models):
+    """Pairs of agents on the same spot: the offset's length is the square
+    root of 0 (the dialect's full IEEE path on those lanes) and
+    float4_normalize_to takes its near-zero branch; bit-exact with the
+    oracle, whose host sqrt is the IEEE one."""
+    recs = W.flock_agents(20000, seed=66, clusters=8, sigma=20.0, box=600.0)
+    recs["p"][1::2] = recs["p"][0::2]
+    print("case: flock with coincident pairs", flush=True)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.flock_model(25.0), recs, 3)
+    assert_bitwise(ga, oa)
+    assert gp == op
