@@ -154,14 +154,15 @@ std::string compose_head(const char *const *sources, unsigned sources_count) {
 }
 
 /* Register budget of the see kernels, as wavefronts per SIMD the compiler
- * must fit.  The plain see kernel asks for 6 (at most 80 VGPRs): its pair
- * loop is issue-bound and hides latency with more wavefronts, and what does
- * not fit is set-up state reloaded once per lane group (1M flocking: 6.1 ->
- * 5.7 ms against the compiler's own 5).  The persistent batch kernel keeps
- * the compiler's choice: its sparse instances walk per lane, scattered loads
- * that more resident wavefronts only make miss more.
- * TAY_SEE_WAVES_PER_EU=<n> (1-8) overrides both, 0 leaves both to the
- * compiler. */
+ * must fit.  The plain see kernel asks for 8 (at most 64 VGPRs): its compact
+ * loop is vector-issue bound and hides its LDS and L2 latency with more
+ * wavefronts, and what does not fit is lane-group set-up state and the
+ * sparse per-lane walk, spilled outside the compact loop (1M flocking, first
+ * steps: 5.25 ms at 6, 5.00 ms at 8; SPH forces 1.20 -> 1.16 ms).  The
+ * persistent batch kernel keeps the compiler's choice: its sparse instances
+ * walk per lane, scattered loads that more resident wavefronts only make
+ * miss more.  TAY_SEE_WAVES_PER_EU=<n> (1-8) overrides both, 0 leaves both
+ * to the compiler. */
 std::string see_occupancy(int fallback) {
     int n = fallback;
     if (const char *w = std::getenv("TAY_SEE_WAVES_PER_EU"))
@@ -174,7 +175,7 @@ std::string see_occupancy(int fallback) {
 std::string compose(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
                     unsigned shapes_count, int count_pairs) {
     std::string text = compose_head(sources, sources_count);
-    const std::string occupancy = see_occupancy(6);
+    const std::string occupancy = see_occupancy(8);
     for (unsigned i = 0; i < shapes_count; ++i) {
         const TayGpuPassShape &p = shapes[i];
         if (p.type == TAY_GPU_PASS_SEE) {
