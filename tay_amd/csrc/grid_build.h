@@ -512,8 +512,6 @@ __global__ __launch_bounds__(kBuildBlock) void k_ranges(const unsigned *__restri
     }
 }
 
-/* Gather every column into key order; word 0 becomes the key (part_i) and
- * word 1 the rank inside the cell (cell_agent_i). */
 /* dst column words [w0, words) of slot i from slot `from` of src, eight
  * loads in flight before their stores (a runtime-length loop would wait on
  * every load before the next). */
@@ -543,6 +541,8 @@ __device__ __forceinline__ void gather_words(const unsigned *__restrict__ src, u
         dst[w * cap + i] = src[w * cap + from];
 }
 
+/* Gather every column into key order; word 0 becomes the key (part_i) and
+ * word 1 the rank inside the cell (cell_agent_i). */
 __global__ __launch_bounds__(kBuildBlock) void k_permute(const unsigned *__restrict__ keys,
                                                          const unsigned *__restrict__ vals, unsigned n,
                                                          unsigned words, unsigned cap, int dims,
