@@ -121,6 +121,20 @@ def test_batch_autotune_restores_agents(gpu):
         assert_bitwise(x, y)
 
 
+def test_batch_of_more_than_256_instances(gpu):
+    """Instances beyond 256 (the batch sort works per instance, whatever
+    their number): still identical to per-instance states."""
+    ens = W.ensemble(260, seed=9, counts=(20, 200), boxes=(60.0, 200.0), radii=(10.0, 30.0))
+    part, tables = parts_and_tables(ens)
+    print("case: batch 260 instances, 2 steps, graph", flush=True)
+    with W.Batch(gpu, ens, part, tables) as b:
+        b.run(2, graph=True)
+        got = b.agents()
+    want, _ = per_instance(gpu, ens, part, tables, 2)
+    for g, w in zip(got, want):
+        assert_bitwise(g, w)
+
+
 def test_batch_rejects_bad_arguments(gpu):
     ens = small_ensemble(2)
     part, tables = parts_and_tables(ens)

@@ -235,3 +235,17 @@ def test_column_index_range_is_checked_at_creation(tay):
     too_many = (1 << 32) // words + 1
     assert not tay.tay_gpu_space_create(api.TAY_GPU_KIND_Z_GRID, A.FLOCK_AGENT.itemsize, too_many, 1, 3, part, 1 << 20)
     assert b"exceeds 2^32 column words" in tay.tay_gpu_last_error()
+
+
+def test_batch_slot_range_is_checked_at_creation(tay):
+    """A batch whose padded slots exceed 2^30 (32-bit byte offsets in a
+    column) is refused before anything is compiled or allocated."""
+    P = api.TayGpuBatchPass
+    srcs = [s.encode() for s in A.source_texts()]
+    arr = (C.c_char_p * len(srcs))(*srcs)
+    passes = (P * 1)(P(1, b"flock_act", 0, A.FLOCK_ACT_CONTEXT.itemsize))
+    counts = (C.c_uint * 2)(1 << 29, (1 << 29) + 1)
+    tables = (C.c_ulonglong * 2)(1 << 20, 1 << 20)
+    assert not tay.tay_gpu_batch_create(arr, len(srcs), passes, 1, A.FLOCK_AGENT.itemsize, 3, 2, counts, tables, 0,
+                                        None, 0)
+    assert b"2^30 slots" in tay.tay_gpu_last_error()
