@@ -86,6 +86,28 @@ def test_forward_matches_the_tay_model(gpu):
     assert np.allclose(out["v"][:, :3], v, rtol=0, atol=1e-4)
 
 
+def test_tay_path_trajectory_within_fp64_drift_bound(gpu, oracle64):
+    """Trajectory drift of the Z-grid path itself: the smooth flocking model
+    (C2 in every step) run through tay.h with TAY_GPU_Z_GRID for 20 steps
+    stays within 1e-4 of the distance travelled of the FP64 trajectory of
+    the same model, on every agent."""
+    p0, v0 = problem(8192, 20, seed=35)
+    rec = np.zeros(len(p0), W.A.SMOOTH_AGENT)
+    rec["p"][:, :3] = p0
+    rec["v"][:, :3] = v0
+    rec["result_index"] = np.arange(len(p0))
+    print("case: tay.h smooth model n=8192 steps=20 vs FP64", flush=True)
+    with W.Simulation(gpu, W.smooth_model(R, tuple(GUESS), H2, DT, DAMP), rec, api.TAY_GPU_Z_GRID) as s:
+        s.run(20)
+        out = s.agents()
+    out = out[np.argsort(out["result_index"])]
+    p64, v64 = forward64(oracle64, p0, v0, GUESS, PRM64, 20)
+    travelled = np.linalg.norm(p64 - p0, axis=1).mean()
+    err = np.abs(out["p"][:, :3].astype(np.float64) - p64).max()
+    assert err <= 1e-4 * travelled, (err, travelled)
+    assert np.abs(out["v"][:, :3].astype(np.float64) - v64).max() <= 1e-4 * np.abs(v64).max()
+
+
 def test_weight_free_motion_closed_form(gpu):
     p0, v0 = problem(2048, 10, seed=33)
     with DiffFlock(gpu, len(p0), R, H2, DT, DAMP) as f:
