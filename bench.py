@@ -71,20 +71,27 @@ def cpu_model():
 
 
 def cpu_baseline(recs, model, threads, steps):
+    """The oracle's CpuGrid (the reported baseline) and CpuZGrid on the same
+    agents, one after the other."""
     from tay_amd import api
     from tay_amd import workloads as W
     oracle = api.TayLib(os.path.join(ROOT, "oracle", "_build", "liboracle.so"))
     models = ctypes.CDLL(os.path.join(ROOT, "oracle", "_build", "libmodels.so"))
     oracle.tay_threads_start(threads, 0)
-    with W.Simulation(oracle, model, recs, api.TAY_CPU_GRID, funcs=models) as s:
-        t0 = time.perf_counter()
-        s.run(steps)
-        wall = time.perf_counter() - t0
-        see_s = oracle.oracle_get_see_seconds(s.state)
-        _, narrow = oracle.pair_counts(s.state, 0)
+    runs = {}
+    for name, space in (("CpuGrid", api.TAY_CPU_GRID), ("CpuZGrid", api.TAY_CPU_Z_GRID)):
+        with W.Simulation(oracle, model, recs, space, funcs=models) as s:
+            t0 = time.perf_counter()
+            s.run(steps)
+            wall = time.perf_counter() - t0
+            see_s = oracle.oracle_get_see_seconds(s.state)
+            _, narrow = oracle.pair_counts(s.state, 0)
+        runs[name] = (narrow / see_s, wall)
     oracle.tay_threads_stop()
+    value, wall = runs["CpuGrid"]
+    z_value, z_wall = runs["CpuZGrid"]
     return {
-        "value": narrow / see_s,
+        "value": value,
         "unit": "pair interactions/s",
         "cores": threads,
         "kind": "port",
@@ -92,6 +99,9 @@ def cpu_baseline(recs, model, threads, steps):
                   "same %d agents; %.1f s wall, %.3f steps/s; host %s, nproc %d"
                   % (steps, len(recs), wall, steps / wall, cpu_model(), os.cpu_count() or 0),
         "steps_per_s": steps / wall,
+        "cpu_z_grid": {"value": z_value, "unit": "pair interactions/s", "steps_per_s": steps / z_wall,
+                       "sample": "the same %d step(s) on CpuZGrid (power-of-two grid, dense Morton cell table); "
+                                 "%.1f s wall" % (steps, z_wall)},
     }
 
 
