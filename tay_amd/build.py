@@ -104,9 +104,25 @@ def build_oracle(with_reference=True):
         _run(["make", "-s", "-C", ORACLE, "ref", "REF=" + REFERENCE])
 
 
+C_HOST = os.path.join(ROOT, "tests", "c_host")
+
+
+def build_c_host():
+    """Build the plain C host test program (tests/c_host/flock_host.c)
+    against include/tay.h and libtay_amd.so, as a reference host links Tay."""
+    out_dir = os.path.join(C_HOST, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "flock_host")
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(PKG, "agent_src"), os.path.join(C_HOST, "flock_host.c"), "-o", out,
+          "-L", PKG, "-ltay_amd", "-Wl,-rpath,$ORIGIN/../../../tay_amd", "-Wl,-rpath-link,/opt/rocm/lib"])
+    return out
+
+
 def build(verbose=False):
     lib = build_library(verbose)
     build_oracle()
+    build_c_host()
     return lib
 
 
