@@ -129,3 +129,39 @@ def test_sph_density_within_fp64_relative(gpu):
     err = np.abs(out["density"].astype(np.float64) - rho64) / rho64
     print("SPH density worst relative error: %.3g" % err.max())
     assert err.max() <= TOL
+
+
+def test_full_size_flocking_positions_counts_and_offsets(gpu):
+    """At the full 1M-agent flocking configuration (64 Gaussian clusters,
+    sigma 30, box 2000, radius 25, cell size 25) with the reference's test
+    model (b_buffer += p_b - p_a, count += 1; test/agent.c:2-8), one see
+    pass: the pair counter equals the sum of the per-seer counts and is even
+    (the box test is symmetric); for 2,000 seers drawn at random every count
+    equals the k-d tree's float32 box-test count and every offset sum meets
+    the FP64 bound."""
+    from tay_amd import agents as A
+    fl = W.flock_agents(1_000_000, seed=1)
+    recs = np.zeros(len(fl), A.AGENT)
+    recs["p"], recs["v"], recs["result_index"] = fl["p"], fl["v"], np.arange(len(fl))
+    r = 25.0
+    with W.Simulation(gpu, see_only(W.test_model(r, part=25.0)), recs, api.TAY_GPU_Z_GRID) as s:
+        gpu.tay_set_pair_counting(s.state, 1)
+        s.run(1)
+        out = s.agents()
+        pairs = gpu.pair_counts(s.state, 0)[1]
+    counts = out["b_buffer_count"].astype(np.int64)
+    assert int(counts.sum()) == pairs and pairs % 2 == 0
+    p = out["p"][:, :3].astype(np.float32)
+    sample = np.random.default_rng(5).choice(len(p), 2000, replace=False)
+    near = cKDTree(p.astype(np.float64)).query_ball_point(p[sample].astype(np.float64), r * (1 + 1e-5), p=np.inf,
+                                                          return_sorted=False)
+    i = np.repeat(sample, [len(x) for x in near])
+    j = np.concatenate([np.asarray(x, dtype=np.int64) for x in near])
+    keep = (i != j) & box_ok(p, i, j, r)
+    i, j = i[keep], j[keep]
+    assert np.array_equal(counts[sample], np.bincount(i, minlength=len(p))[sample])
+    rows = np.searchsorted(np.sort(sample), i)
+    order = np.argsort(sample)
+    worst = check_sum(out["b_buffer"][sample[order], :3], p[j].astype(np.float64) - p[i], rows, len(sample),
+                      "b_buffer")
+    print("1M flocking positions: %d pairs, sampled offsets worst relative to sum|terms|: %.3g" % (pairs, worst))
