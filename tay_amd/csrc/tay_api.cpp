@@ -313,6 +313,13 @@ TAY_API TayGroup *tay_add_group(TayState *state, unsigned agent_size, unsigned a
     TayGroup *g = state->groups + gi;
     std::memset(g, 0, sizeof(*g));
     g->storage = (char *)std::calloc(agent_capacity ? agent_capacity : 1, agent_size ? agent_size : 1);
+    if (!g->storage) {
+        char msg[96];
+        std::snprintf(msg, sizeof(msg), "cannot allocate host storage for %u agents of %u bytes", agent_capacity,
+                      agent_size);
+        set_error(state, TAY_ERROR_CAPACITY, msg);
+        return nullptr;
+    }
     g->agent_size = agent_size;
     g->capacity = agent_capacity;
     g->is_point = is_point;
@@ -523,7 +530,12 @@ TAY_API void tay_log(void *file, char *fmt, ...) {
 /* tay/ocl.c:274-282 */
 TAY_API void ocl_add_source(TayState *state, const char *source) {
     if (state->sources_count < kMaxSources && source) {
-        state->sources[state->sources_count++] = strdup(source);
+        char *copy = strdup(source);
+        if (!copy) {
+            set_error(state, TAY_ERROR_CAPACITY, "cannot copy a model source");
+            return;
+        }
+        state->sources[state->sources_count++] = copy;
         state->recompile = 1;
     }
 }

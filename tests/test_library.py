@@ -215,6 +215,17 @@ def test_capacity_and_group_slots(tay):
     tay.tay_destroy_state(st)
 
 
+def test_host_storage_that_cannot_be_allocated_is_an_error(tay):
+    """2^31 agents of 64 KiB (128 TiB) cannot be allocated: the group is
+    refused with TAY_ERROR_CAPACITY instead of handing out a null storage."""
+    st = tay.tay_create_state()
+    assert not tay.tay_add_group(st, 1 << 16, 1 << 31, 1)
+    assert tay.tay_get_error(st) == api.TAY_ERROR_CAPACITY
+    assert "cannot allocate host storage" in tay.error_message(st)
+    assert tay.tay_add_group(st, 32, 4, 1)
+    tay.tay_destroy_state(st)
+
+
 @no_gpu
 def test_no_device_is_reported(tay):
     st, g = _state_with_group(tay, api.TAY_GPU_Z_GRID)
