@@ -151,6 +151,20 @@ __device__ __forceinline__ unsigned tay_shift_in_ngt(unsigned bits, float m, flo
     return bits;
 }
 
+/* Wavefront maximum of a value below 128, from the top bit down: one
+ * compare and one ballot per bit, no cross-lane data movement.  Every lane
+ * of the wavefront must be active. */
+__device__ __forceinline__ unsigned tay_wave_max_u7(unsigned v) {
+    unsigned r = 0;
+#pragma unroll
+    for (int b = 6; b >= 0; --b) {
+        const unsigned t = r | (1u << b);
+        if (__ballot(v >= t) != 0ull)
+            r = t;
+    }
+    return r;
+}
+
 __device__ __forceinline__ unsigned tay_wave_max_u(unsigned v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -302,28 +316,54 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         if (COUNT)
             pairs += (unsigned)__popcll(mask);
 
-        const unsigned rounds = tay_wave_max_u((unsigned)__popcll(mask));
-        for (unsigned round = 0; round < rounds; ++round) {
-            /* lowest set bit and its removal computed on every lane (an empty
-             * mask stays empty), so the mask needs no merge after the body */
-            const bool has = mask != 0ull;
-            const unsigned c = (unsigned)__builtin_ctzll(mask) & 63u;
-            mask &= mask - 1ull;
-            if (has) {
-                const float4 rec = *(const float4 *)chunk_lo[wave][c];
-                union {
-                    typename T::seen agent;
-                    unsigned w[SEEN_WORDS];
-                } B;
-                const unsigned off = __float_as_uint(rec.w);
+        auto visit = [&](unsigned c) {
+            const float4 rec = *(const float4 *)chunk_lo[wave][c];
+            union {
+                typename T::seen agent;
+                unsigned w[SEEN_WORDS];
+            } B;
+            const unsigned off = __float_as_uint(rec.w);
 #pragma unroll
-                for (unsigned k = 0; k < SEEN_WORDS; ++k)
-                    B.w[k] = *(const unsigned *)(seen_col[k] + off);
-                const float pos[3] = {rec.x, rec.y, rec.z};
+            for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                B.w[k] = *(const unsigned *)(seen_col[k] + off);
+            const float pos[3] = {rec.x, rec.y, rec.z};
 #pragma unroll
-                for (int k = 0; k < DIMS; ++k)
-                    B.w[2 + k] = __float_as_uint(pos[k]);
-                call(&A.agent, &B.agent, ctx);
+            for (int k = 0; k < DIMS; ++k)
+                B.w[2 + k] = __float_as_uint(pos[k]);
+            call(&A.agent, &B.agent, ctx);
+        };
+        /* The lowest set bit and its removal are computed on every lane (an
+         * empty mask stays empty), so the mask needs no merge after the
+         * body.  A 32-bit mask costs one find-first-bit and two operations
+         * to clear per round, a 64-bit one about ten; walking the two
+         * halves one after the other keeps the ascending order but takes
+         * max(lo) + max(hi) rounds instead of max(lo + hi), so it is used
+         * when that adds at most a quarter of the rounds (dense chunks). */
+        const unsigned lo_m = (unsigned)mask, hi_m = (unsigned)(mask >> 32);
+        const unsigned n_lo = (unsigned)__popc(lo_m), n_hi = (unsigned)__popc(hi_m);
+        const unsigned r_lo = tay_wave_max_u7(n_lo), r_hi = tay_wave_max_u7(n_hi);
+        const unsigned r_all = tay_wave_max_u7(n_lo + n_hi);
+        if ((r_lo + r_hi) * 4u <= r_all * 5u) {
+#pragma unroll 1
+            for (unsigned h = 0; h < 2u; ++h) {
+                unsigned m = h ? hi_m : lo_m;
+                const unsigned base = 32u * h;
+                const unsigned rounds = h ? r_hi : r_lo;
+                for (unsigned round = 0; round < rounds; ++round) {
+                    const bool has = m != 0u;
+                    const unsigned c = base + ((unsigned)__builtin_ctz(m) & 31u);
+                    m &= m - 1u;
+                    if (has)
+                        visit(c);
+                }
+            }
+        } else {
+            for (unsigned round = 0; round < r_all; ++round) {
+                const bool has = mask != 0ull;
+                const unsigned c = (unsigned)__builtin_ctzll(mask) & 63u;
+                mask &= mask - 1ull;
+                if (has)
+                    visit(c);
             }
         }
         tay_wave_sync();
