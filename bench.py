@@ -45,7 +45,7 @@ B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
 # inside this process, so the figure is carried here and re-measured whenever
 # the see kernel changes.
 SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0,
-               "bytes": int((81911.2 * 2.000 + 162035.9 * 1.000) * 1024)}
+               "bytes": int((96527.0 * 2.000 + 216079.0 * 1.000) * 1024)}
 
 
 def advance(sim, steps, name):
