@@ -17,6 +17,8 @@
 #ifndef tay_gpu_h
 #define tay_gpu_h
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -43,6 +45,14 @@ const char *tay_gpu_last_error(void);
 
 /* Waits for all queued device work.  Replaces ocl_finish (tay/ocl.h:94). */
 int tay_gpu_synchronize(void);
+
+/* Zeroed page-locked host memory of `bytes` (at most 4 GiB), for agent
+ * storage that tay_gpu_space_push / tay_gpu_space_fetch copy at the link's
+ * full rate; NULL without a GPU or when the allocation fails, and the
+ * caller then uses ordinary memory.  Replaces the calloc of a group's
+ * storage (tay/state.c:88-89).  Free with tay_gpu_host_free. */
+void *tay_gpu_host_alloc(size_t bytes);
+void tay_gpu_host_free(void *p);
 
 /* ------------------------------------------------------------- spaces */
 

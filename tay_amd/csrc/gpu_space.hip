@@ -107,6 +107,27 @@ TAY_API int tay_gpu_available(void) {
     return stream() != nullptr;
 }
 
+/* Page-locked host memory is handed out up to this size per allocation;
+ * larger host storage stays pageable. */
+constexpr size_t kMaxPinnedBytes = 4ull << 30;
+
+TAY_API void *tay_gpu_host_alloc(size_t bytes) {
+    if (bytes == 0 || bytes > kMaxPinnedBytes || !stream())
+        return nullptr;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::memset(p, 0, bytes);
+    return p;
+}
+
+TAY_API void tay_gpu_host_free(void *p) {
+    if (p)
+        (void)hipHostFree(p);
+}
+
 TAY_API const char *tay_gpu_last_error(void) {
     return g_last_error.c_str();
 }

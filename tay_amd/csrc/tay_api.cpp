@@ -32,6 +32,7 @@ constexpr unsigned kLibraryDefault = ~0u;  /* a tuning knob left at the GPU libr
 
 struct TayGroup {
     char *storage;
+    int storage_pinned;         /* storage from tay_gpu_host_alloc, else calloc */
     unsigned agent_size;
     unsigned capacity;
     int is_point;
@@ -267,7 +268,10 @@ TAY_API void tay_destroy_state(TayState *state) {
         return;
     release_device(state);
     for (TayGroup &g : state->groups)
-        std::free(g.storage);
+        if (g.storage_pinned)
+            tay_gpu_host_free(g.storage);
+        else
+            std::free(g.storage);
     for (unsigned i = 0; i < state->sources_count; ++i)
         std::free(state->sources[i]);
     delete state;
@@ -312,7 +316,13 @@ TAY_API TayGroup *tay_add_group(TayState *state, unsigned agent_size, unsigned a
     }
     TayGroup *g = state->groups + gi;
     std::memset(g, 0, sizeof(*g));
-    g->storage = (char *)std::calloc(agent_capacity ? agent_capacity : 1, agent_size ? agent_size : 1);
+    /* page-locked when a GPU is present, so fetches and pushes copy at the
+     * link's full rate; ordinary zeroed memory otherwise */
+    const size_t bytes = (size_t)(agent_capacity ? agent_capacity : 1) * (size_t)(agent_size ? agent_size : 1);
+    g->storage = (char *)tay_gpu_host_alloc(bytes);
+    g->storage_pinned = g->storage != nullptr;
+    if (!g->storage)
+        g->storage = (char *)std::calloc(agent_capacity ? agent_capacity : 1, agent_size ? agent_size : 1);
     if (!g->storage) {
         char msg[96];
         std::snprintf(msg, sizeof(msg), "cannot allocate host storage for %u agents of %u bytes", agent_capacity,
