@@ -84,7 +84,8 @@ int tay_gpu_space_unfix_box(TayGpuSpace *space);
 int tay_gpu_space_set_order_bits(TayGpuSpace *space, int bits);
 
 /* Copies count AoS records from host memory to the device and makes them the
- * space's agents.  Replaces ocl_push_agents_non_blocking (tay/ocl.c:238-250). */
+ * space's agents; the host records may change once it returns.  Replaces
+ * ocl_push_agents_non_blocking (tay/ocl.c:238-250). */
 int tay_gpu_space_push(TayGpuSpace *space, const void *agents, unsigned count);
 
 /* Copies the space's agents, in current storage order, to count AoS records

@@ -297,7 +297,9 @@ TAY_API int tay_gpu_space_push(TayGpuSpace *sp, const void *agents, unsigned cou
         return 0;
     k_aos_to_soa<<<blocks_for((unsigned long long)count * sp->words, kBuildBlock, 8192), kBuildBlock, 0, s>>>(
         sp->staging, sp->cols[sp->cur], count, sp->words, sp->capacity);
-    return launch_ok("aos_to_soa");
+    /* from page-locked storage the copy runs asynchronously: wait for it,
+     * since the caller may change the records as soon as this returns */
+    return launch_ok("aos_to_soa") && hip_ok(hipStreamSynchronize(s), "push agents");
 }
 
 TAY_API int tay_gpu_space_fetch(TayGpuSpace *sp, void *agents, unsigned count) {
