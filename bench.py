@@ -334,7 +334,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: the Z-grid runs on a single MI355X in this build; "
+        print("bench.py measures one GPU: the Z-grid space runs on a single MI355X; "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
 
