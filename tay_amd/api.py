@@ -117,6 +117,8 @@ _GPU = {
     "tay_gpu_available": (_i, []),
     "tay_gpu_last_error": (C.c_char_p, []),
     "tay_gpu_synchronize": (_i, []),
+    "tay_gpu_host_alloc": (_vp, [C.c_size_t]),
+    "tay_gpu_host_free": (None, [_vp]),
     "tay_gpu_copy_probe": (_i, [C.c_ulonglong, C.POINTER(C.c_double)]),
     "tay_gpu_sqrt_selftest": (_i, [C.c_ulonglong, C.c_ulonglong, C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint)]),
     "tay_gpu_space_create": (_vp, [_i, _u, _u, _i, _i, C.POINTER(C.c_float), C.c_ulonglong]),

@@ -264,3 +264,16 @@ def test_flocking_coincident_agents_bit_exact(gpu, oracle, models):
     ga, oa, gp, op = run_pair(gpu, oracle, models, W.flock_model(25.0), recs, 3)
     assert_bitwise(ga, oa)
     assert gp == op
+
+
+def test_page_locked_host_storage(gpu):
+    """Group storage comes from tay_gpu_host_alloc on a GPU host: zeroed,
+    at most 4 GiB per allocation, and agents fetched into it after a run
+    equal the oracle's (the other tests of this file cover the values)."""
+    import ctypes as C
+    p = gpu.tay_gpu_host_alloc(1 << 20)
+    assert p
+    assert bytes((C.c_char * (1 << 20)).from_address(p)) == bytes(1 << 20)
+    gpu.tay_gpu_host_free(p)
+    assert not gpu.tay_gpu_host_alloc((4 << 30) + 1)
+    assert not gpu.tay_gpu_host_alloc(0)

@@ -260,3 +260,15 @@ def test_batch_slot_range_is_checked_at_creation(tay):
     assert not tay.tay_gpu_batch_create(arr, len(srcs), passes, 1, A.FLOCK_AGENT.itemsize, 3, 2, counts, tables, 0,
                                         None, 0)
     assert b"2^30 slots" in tay.tay_gpu_last_error()
+
+
+@no_gpu
+def test_page_locked_storage_needs_a_gpu(tay):
+    """Without a GPU the page-locked allocator returns NULL and a group's
+    storage is ordinary zeroed memory."""
+    assert not tay.tay_gpu_host_alloc(1 << 20)
+    st = tay.tay_create_state()
+    g = tay.tay_add_group(st, 32, 8, 1)
+    p = tay.tay_get_available_agent(st, g)
+    assert p and bytes((C.c_char * 256).from_address(p)) == bytes(256)
+    tay.tay_destroy_state(st)
