@@ -19,9 +19,10 @@ constexpr unsigned kBuildBlock = 256;
 constexpr unsigned kRadixBits = 8;
 constexpr unsigned kRadix = 1u << kRadixBits;
 constexpr unsigned kRadixPasses = 4;            /* 32-bit keys */
-constexpr unsigned kRadixWaves = kBuildBlock / 64;
-constexpr unsigned kRadixRounds = 16;
-constexpr unsigned kRadixTile = kRadixWaves * kRadixRounds * 64;   /* keys per block */
+constexpr unsigned kRadixTile = 4096;              /* keys per tile (one block) */
+constexpr unsigned kRadixBlock = 1024;             /* threads of the histogram and scatter blocks */
+constexpr unsigned kRadixWaves = kRadixBlock / 64;
+constexpr unsigned kRadixRounds = kRadixTile / kRadixBlock;   /* 64-key rounds per wavefront */
 constexpr unsigned kScanBlocks = 64;             /* segments of the histogram scan, at most one wavefront's lanes */
 
 __device__ __forceinline__ unsigned compact2(unsigned v) {
@@ -315,17 +316,19 @@ __device__ __forceinline__ TaySortTile sort_tile(const TaySortTile *__restrict__
 }
 
 /* Digit histogram of one tile of at most 4096 keys per block. */
-__global__ __launch_bounds__(kBuildBlock) void k_radix_hist(const unsigned *__restrict__ keys, unsigned n,
+__global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const unsigned *__restrict__ keys, unsigned n,
                                                             unsigned shift, unsigned *__restrict__ hist,
                                                             unsigned nblocks, const TaySortTile *__restrict__ seg) {
     __shared__ unsigned h[kRadix];
-    h[threadIdx.x] = 0;
+    if (threadIdx.x < kRadix)
+        h[threadIdx.x] = 0;
     __syncthreads();
     const TaySortTile tile = sort_tile(seg, blockIdx.x, n, nblocks);
-    for (unsigned r = threadIdx.x; r < tile.count; r += kBuildBlock)
+    for (unsigned r = threadIdx.x; r < tile.count; r += kRadixBlock)
         atomicAdd(&h[(keys[tile.start + r] >> shift) & (kRadix - 1)], 1u);
     __syncthreads();
-    hist[tile.hist_base + threadIdx.x * tile.hist_stride] = h[threadIdx.x];
+    if (threadIdx.x < kRadix)
+        hist[tile.hist_base + threadIdx.x * tile.hist_stride] = h[threadIdx.x];
 }
 
 /* Exclusive scan of the digit-major histogram in place, in three steps over
@@ -398,14 +401,15 @@ __global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict_
     }
 }
 
-/* Stable scatter of one tile, through LDS.  Wave w owns the contiguous
- * quarter [w * 1024, (w + 1) * 1024) of the tile; inside a wave, 16 rounds
- * of 64 keys go in order and lanes with equal digits are ranked by lane
- * (ballot match).  Every key first goes to its place in the tile sorted by
+/* Stable scatter of one tile, through LDS.  Wave w of the 16 owns the
+ * contiguous sixteenth [w * 256, (w + 1) * 256) of the tile; inside a wave,
+ * 4 rounds of 64 keys go in order and lanes with equal digits are ranked by
+ * lane (ballot match).  Sixteen wavefronts per tile keep enough loads in
+ * flight when the tiles are few (a 1M-key sort has 245).  Every key first goes to its place in the tile sorted by
  * digit (in LDS); then the block writes the sorted tile out in order, so
  * the keys of one digit leave as one contiguous run of the output instead
  * of one scattered word each. */
-__global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *__restrict__ keys_in,
+__global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const unsigned *__restrict__ keys_in,
                                                                const unsigned *__restrict__ vals_in,
                                                                unsigned *__restrict__ keys_out,
                                                                unsigned *__restrict__ vals_out, unsigned n,
@@ -415,10 +419,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
     __shared__ unsigned wave_hist[kRadixWaves][kRadix];
     __shared__ unsigned lbase[kRadixWaves][kRadix];   /* next tile slot of (wave, digit) */
     __shared__ unsigned delta[kRadix];                /* output slot minus tile slot of a digit */
-    __shared__ unsigned wave_sum[kRadixWaves];
+    __shared__ unsigned wave_sum[kRadix / 64];
     __shared__ unsigned lkeys[kRadixTile], lvals[kRadixTile];
     const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    for (unsigned k = t; k < kRadixWaves * kRadix; k += kBuildBlock)
+    for (unsigned k = t; k < kRadixWaves * kRadix; k += kRadixBlock)
         (&wave_hist[0][0])[k] = 0;
     __syncthreads();
     const TaySortTile tile = sort_tile(seg, blockIdx.x, n, nblocks);
@@ -430,16 +434,18 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
             atomicAdd(&wave_hist[w][(keys_in[idx] >> shift) & (kRadix - 1)], 1u);
     }
     __syncthreads();
-    {
-        /* thread t is digit t: its count in the tile, the tile slots before
-         * it (scan over the digits), then per wave */
-        unsigned c = 0;
+    /* threads t < kRadix: thread t is digit t, with its count in the tile,
+     * the tile slots before it (scan over the digits), then per wave */
+    unsigned c = 0, inc = 0;
+    if (t < kRadix) {
         for (unsigned ww = 0; ww < kRadixWaves; ++ww)
             c += wave_hist[ww][t];
-        const unsigned inc = wave_inclusive_sum(c, lane);
+        inc = wave_inclusive_sum(c, lane);
         if (lane == 63u)
             wave_sum[w] = inc;
-        __syncthreads();
+    }
+    __syncthreads();
+    if (t < kRadix) {
         unsigned start = inc - c;
         for (unsigned ww = 0; ww < w; ++ww)
             start += wave_sum[ww];
@@ -476,7 +482,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_radix_scatter(const unsigned *_
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    for (unsigned p = t; p < tile.count; p += kBuildBlock) {
+    for (unsigned p = t; p < tile.count; p += kRadixBlock) {
         const unsigned key = lkeys[p], val = lvals[p];
         const unsigned pos = delta[(key >> shift) & (kRadix - 1)] + p;
         keys_out[pos] = key;
