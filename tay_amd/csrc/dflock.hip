@@ -484,7 +484,6 @@ void df_sort(TayDflock *f, const float *cols, hipStream_t s) {
         k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(f->keys[in], n, pass * kRadixBits, f->block_hist, nblocks,
                                                      nullptr);
         k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(f->block_hist, total, f->scan_sums);
-        k_scan_top<<<1, 64, 0, s>>>(f->scan_sums, segs);
         k_scan_down<<<segs, kBuildBlock, 0, s>>>(f->block_hist, total, f->scan_sums);
         k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(f->keys[in], f->vals[in], f->keys[out], f->vals[out], n,
                                                         pass * kRadixBits, f->block_hist, nblocks, pass == 0, nullptr);

@@ -420,7 +420,6 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
         const unsigned in = pass & 1u, out = in ^ 1u;
         k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(b->keys[in], n, shift, b->block_hist, nblocks, b->sort_seg);
         k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums);
-        k_scan_top<<<1, 64, 0, s>>>(b->scan_sums, segs);
         k_scan_down<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums);
         k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(b->keys[in], b->vals[in], b->keys[out], b->vals[out], n,
                                                         shift, b->block_hist, nblocks, pass == 0, b->sort_seg);

@@ -331,9 +331,10 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const unsigned *__re
         hist[tile.hist_base + threadIdx.x * tile.hist_stride] = h[threadIdx.x];
 }
 
-/* Exclusive scan of the digit-major histogram in place, in three steps over
- * kScanBlocks contiguous segments: segment sums, a one-wavefront scan of the
- * sums, then each segment scanned in coalesced rounds from its carry-in. */
+/* Exclusive scan of the digit-major histogram in place, in two steps over
+ * kScanBlocks contiguous segments: segment sums, then each segment scanned
+ * in coalesced rounds from its carry-in (the sum of the segment sums before
+ * it, formed by each block). */
 __device__ __forceinline__ unsigned wave_inclusive_sum(unsigned v, unsigned lane) {
     for (unsigned o = 1; o < 64; o <<= 1) {
         unsigned u = __shfl_up(v, o);
@@ -367,13 +368,6 @@ __global__ __launch_bounds__(kBuildBlock) void k_scan_reduce(const unsigned *__r
     }
 }
 
-__global__ __launch_bounds__(64) void k_scan_top(unsigned *__restrict__ sums, unsigned n) {
-    const unsigned lane = threadIdx.x;
-    const unsigned v = lane < n ? sums[lane] : 0u;
-    const unsigned inc = wave_inclusive_sum(v, lane);
-    if (lane < n)
-        sums[lane] = inc - v;
-}
 
 __global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict__ hist, unsigned total,
                                                            const unsigned *__restrict__ sums) {
@@ -381,7 +375,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict_
     const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const unsigned beg = segment_begin(blockIdx.x, total, gridDim.x);
     const unsigned end = segment_begin(blockIdx.x + 1, total, gridDim.x);
-    unsigned carry = sums[blockIdx.x];
+    /* carry-in: the sums of the segments before this one (at most 64, one
+     * per lane), added up by every wavefront */
+    unsigned carry = wave_inclusive_sum(lane < blockIdx.x ? sums[lane] : 0u, lane);
+    carry = __shfl(carry, 63);
     for (unsigned base = beg; base < end; base += kBuildBlock) {
         const unsigned i = base + threadIdx.x;
         const unsigned v = i < end ? hist[i] : 0u;
