@@ -340,18 +340,24 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
     k_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, sp->dims, sp->geom, sp->keys[0]);
 
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
-    for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
-        const unsigned shift = pass * kRadixBits;
-        const unsigned in = pass & 1u, out = in ^ 1u;
-        const unsigned total = kRadix * nblocks;
-        const unsigned segs = total < kScanBlocks * kBuildBlock ? (total + kBuildBlock - 1) / kBuildBlock : kScanBlocks;
-        k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], n, shift, sp->block_hist, nblocks, nullptr);
-        k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
-        k_scan_down<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
-        k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], sp->vals[in], sp->keys[out], sp->vals[out], n,
-                                                        shift, sp->block_hist, nblocks, pass == 0, nullptr);
+    if (n <= kSmallSort) {
+        k_small_sort<<<1, kRadixBlock, 0, s>>>(sp->keys[0], sp->vals[0], n);
+    } else {
+        for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
+            const unsigned shift = pass * kRadixBits;
+            const unsigned in = pass & 1u, out = in ^ 1u;
+            const unsigned total = kRadix * nblocks;
+            const unsigned segs =
+                total < kScanBlocks * kBuildBlock ? (total + kBuildBlock - 1) / kBuildBlock : kScanBlocks;
+            k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], n, shift, sp->block_hist, nblocks, nullptr);
+            k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
+            k_scan_down<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
+            k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], sp->vals[in], sp->keys[out], sp->vals[out],
+                                                            n, shift, sp->block_hist, nblocks, pass == 0, nullptr);
+        }
     }
-    /* kRadixPasses is even: the sorted keys and source indices are back in buffer 0 */
+    /* kRadixPasses is even: the sorted keys and source indices are back in
+     * buffer 0 (the small sort works in place there) */
     k_ranges_clear<<<blocks_for(sp->max_cells, kBuildBlock, 4096), kBuildBlock, 0, s>>>(sp->ranges, sp->geom);
     k_ranges<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], n, sp->dims, sp->geom, sp->ranges);
     k_permute<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], sp->vals[0], n, sp->words, cap, sp->dims, sp->geom,

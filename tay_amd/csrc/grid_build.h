@@ -487,6 +487,97 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const unsigned *_
     }
 }
 
+/* Stable LSD radix sort of at most kSmallSort keys by one block, all four
+ * passes in LDS: one launch instead of four histogram, scan and scatter
+ * rounds, for the small states whose step is a chain of short launches.
+ * Wave w owns the contiguous slice [w * per, (w + 1) * per) of the current
+ * order; per pass each lane keeps its keys in registers, ranks them inside
+ * the wave round by round (ballot match, as k_radix_scatter), the digit
+ * starts are scanned over (digit, wave), and the keys are written back to
+ * the LDS arrays at their new places.  The source index of slot i starts as
+ * i.  Sorts keys[0, n) in place and writes the source indices to vals. */
+constexpr unsigned kSmallSort = 16384;
+constexpr unsigned kSmallPerLane = kSmallSort / kRadixBlock;   /* keys held by one lane */
+
+__global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict__ keys,
+                                                            unsigned *__restrict__ vals, unsigned n) {
+    __shared__ unsigned lkeys[kSmallSort], lvals[kSmallSort];
+    __shared__ unsigned cnt[kRadixWaves][kRadix];
+    __shared__ unsigned wave_sum[kRadix / 64];
+    const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    for (unsigned i = t; i < n; i += kRadixBlock) {
+        lkeys[i] = keys[i];
+        lvals[i] = i;
+    }
+    const unsigned rounds = (n + kRadixWaves * 64u - 1u) / (kRadixWaves * 64u);   /* <= kSmallPerLane */
+    const unsigned per = rounds * 64u;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
+        const unsigned shift = pass * kRadixBits;
+        for (unsigned k = t; k < kRadixWaves * kRadix; k += kRadixBlock)
+            (&cnt[0][0])[k] = 0;
+        __syncthreads();
+        unsigned key[kSmallPerLane], val[kSmallPerLane], pos[kSmallPerLane];
+#pragma unroll
+        for (unsigned r = 0; r < kSmallPerLane; ++r) {
+            const unsigned idx = w * per + r * 64u + lane;
+            const bool valid = r < rounds && idx < n;
+            key[r] = valid ? lkeys[idx] : 0u;
+            val[r] = valid ? lvals[idx] : 0u;
+            const unsigned d = (key[r] >> shift) & (kRadix - 1);
+            unsigned long long m = __ballot(valid);
+#pragma unroll
+            for (unsigned b = 0; b < kRadixBits; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const unsigned long long vote = __ballot(bit);
+                m &= bit ? vote : ~vote;
+            }
+            /* slot inside the wave's run of digit d, then the run grows */
+            pos[r] = valid ? cnt[w][d] + (unsigned)__popcll(m & below) : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (valid && (m & below) == 0ull)
+                cnt[w][d] += (unsigned)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        /* threads t < kRadix: digit t's start, then each wave's run start */
+        unsigned c = 0, inc = 0;
+        if (t < kRadix) {
+            for (unsigned ww = 0; ww < kRadixWaves; ++ww)
+                c += cnt[ww][t];
+            inc = wave_inclusive_sum(c, lane);
+            if (lane == 63u)
+                wave_sum[w] = inc;
+        }
+        __syncthreads();
+        if (t < kRadix) {
+            unsigned start = inc - c;
+            for (unsigned ww = 0; ww < w; ++ww)
+                start += wave_sum[ww];
+            for (unsigned ww = 0; ww < kRadixWaves; ++ww) {
+                const unsigned run = cnt[ww][t];
+                cnt[ww][t] = start;
+                start += run;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (unsigned r = 0; r < kSmallPerLane; ++r) {
+            const unsigned idx = w * per + r * 64u + lane;
+            if (r < rounds && idx < n) {
+                const unsigned dst = cnt[w][(key[r] >> shift) & (kRadix - 1)] + pos[r];
+                lkeys[dst] = key[r];
+                lvals[dst] = val[r];
+            }
+        }
+        __syncthreads();
+    }
+    for (unsigned i = t; i < n; i += kRadixBlock) {
+        keys[i] = lkeys[i];
+        vals[i] = lvals[i];
+    }
+}
+
 __global__ __launch_bounds__(kBuildBlock) void k_ranges_clear(TayCellRange *__restrict__ ranges,
                                                               const TayGridGeom *__restrict__ g) {
     const unsigned long long cells = g->cells;
