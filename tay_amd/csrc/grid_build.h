@@ -494,8 +494,9 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const unsigned *_
  * order; per pass each lane keeps its keys in registers, ranks them inside
  * the wave round by round (ballot match, as k_radix_scatter), the digit
  * starts are scanned over (digit, wave), and the keys are written back to
- * the LDS arrays at their new places.  The source index of slot i starts as
- * i.  Sorts keys[0, n) in place and writes the source indices to vals. */
+ * the LDS arrays at their new places; passes above the largest key's top
+ * digit are skipped.  The source index of slot i starts as i.  Sorts
+ * keys[0, n) in place and writes the source indices to vals. */
 constexpr unsigned kSmallSort = 16384;
 constexpr unsigned kSmallPerLane = kSmallSort / kRadixBlock;   /* keys held by one lane */
 
@@ -504,16 +505,35 @@ __global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict
     __shared__ unsigned lkeys[kSmallSort], lvals[kSmallSort];
     __shared__ unsigned cnt[kRadixWaves][kRadix];
     __shared__ unsigned wave_sum[kRadix / 64];
+    __shared__ unsigned wave_max_key[kRadixWaves];
     const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    unsigned kmax = 0;
     for (unsigned i = t; i < n; i += kRadixBlock) {
-        lkeys[i] = keys[i];
+        const unsigned k = keys[i];
+        lkeys[i] = k;
         lvals[i] = i;
+        kmax = k > kmax ? k : kmax;
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned u = __shfl_xor(kmax, o);
+        kmax = u > kmax ? u : kmax;
+    }
+    if (lane == 0u)
+        wave_max_key[w] = kmax;
+    __syncthreads();
+    kmax = 0;
+    for (unsigned ww = 0; ww < kRadixWaves; ++ww)
+        kmax = wave_max_key[ww] > kmax ? wave_max_key[ww] : kmax;
     const unsigned rounds = (n + kRadixWaves * 64u - 1u) / (kRadixWaves * 64u);   /* <= kSmallPerLane */
     const unsigned per = rounds * 64u;
     const unsigned long long below = (1ull << lane) - 1ull;
     for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
         const unsigned shift = pass * kRadixBits;
+        /* every digit from here up is zero: the remaining passes would keep
+         * the order (stable), so the keys are sorted */
+        if (pass > 0u && (kmax >> shift) == 0u)
+            break;
         for (unsigned k = t; k < kRadixWaves * kRadix; k += kRadixBlock)
             (&cnt[0][0])[k] = 0;
         __syncthreads();
