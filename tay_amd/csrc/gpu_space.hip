@@ -337,7 +337,9 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
                                 sp->max_cells, sp->order_bits, sp->box_fixed, sp->fixed_min[0], sp->fixed_min[1],
                                 sp->fixed_min[2], sp->fixed_max[0], sp->fixed_max[1], sp->fixed_max[2], sp->geom);
     const unsigned agent_blocks = blocks_for(n, kBuildBlock, 0xFFFFFFFFu);
-    k_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, sp->dims, sp->geom, sp->keys[0]);
+    const unsigned clear_blocks = blocks_for(sp->max_cells, kBuildBlock, 4096);
+    k_keys_clear<<<agent_blocks > clear_blocks ? agent_blocks : clear_blocks, kBuildBlock, 0, s>>>(
+        src, cap, n, sp->dims, sp->geom, sp->keys[0], sp->ranges);
 
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
     if (n <= kSmallSort) {
@@ -358,7 +360,6 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
     }
     /* kRadixPasses is even: the sorted keys and source indices are back in
      * buffer 0 (the small sort works in place there) */
-    k_ranges_clear<<<blocks_for(sp->max_cells, kBuildBlock, 4096), kBuildBlock, 0, s>>>(sp->ranges, sp->geom);
     k_ranges<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], n, sp->dims, sp->geom, sp->ranges);
     k_permute<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], sp->vals[0], n, sp->words, cap, sp->dims, sp->geom,
                                                    sp->ranges, src, dst);

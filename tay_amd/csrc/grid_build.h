@@ -291,6 +291,23 @@ __global__ __launch_bounds__(kBuildBlock) void k_keys(const unsigned *__restrict
     keys[i] = storage_key(cols, cap, i, dims, g);
 }
 
+/* k_keys and k_ranges_clear in one launch: slot keys for i < n, and the
+ * step's cell table cleared by a grid-stride loop (nothing reads the table
+ * between here and k_ranges). */
+__global__ __launch_bounds__(kBuildBlock) void k_keys_clear(const unsigned *__restrict__ cols, unsigned cap,
+                                                            unsigned n, int dims, const TayGridGeom *__restrict__ g,
+                                                            unsigned *__restrict__ keys,
+                                                            TayCellRange *__restrict__ ranges) {
+    const unsigned i = blockIdx.x * kBuildBlock + threadIdx.x;
+    if (i < n)
+        keys[i] = storage_key(cols, cap, i, dims, g);
+    const unsigned long long cells = g->cells;
+    for (unsigned long long c = i; c < cells; c += (unsigned long long)gridDim.x * kBuildBlock) {
+        ranges[c].beg = 0;
+        ranges[c].end = 0;
+    }
+}
+
 /* One radix tile of a segmented sort: keys [start, start + count) never
  * leave their segment; the tile's count of digit d goes to histogram word
  * hist_base + d * hist_stride.  Laid out segment-major (for each segment,
