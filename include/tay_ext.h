@@ -66,8 +66,18 @@ typedef enum TayFetchMode {
 void tay_set_fetch_mode(TayState *state, int mode);
 
 /* Copies the group's agents to host storage now (all groups when group is
- * NULL); returns 1 on success. */
+ * NULL); returns 1 on success.  Before the first tay_run and after
+ * tay_simulation_end the host storage already holds the newest agents
+ * (the end of a simulation fetches them under TAY_FETCH_ON_DEMAND). */
 int tay_fetch_agents(TayState *state, TayGroup *group);
+
+/* Copies bytes [offset, offset + size) of each of the group's agents, in
+ * current storage order, into dst as packed records of `size` bytes (for
+ * example offset 8, size 12: the positions), leaving host storage alone.
+ * offset and size are multiples of 4 within the agent size; the group must
+ * be on the device (after a tay_run).  Returns the number of records
+ * copied, 0 on error. */
+unsigned tay_fetch_agent_fields(TayState *state, TayGroup *group, unsigned offset, unsigned size, void *dst);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,15 @@ int tay_gpu_space_push(TayGpuSpace *space, const void *agents, unsigned count);
  * in host memory.  Replaces ocl_fetch_agents (tay/ocl.c:261-272). */
 int tay_gpu_space_fetch(TayGpuSpace *space, void *agents, unsigned count);
 
+/* Copies words [first_word, first_word + words) of the space's first count
+ * agents, in current storage order, to host memory as count packed records
+ * of 4 * words bytes: the selected fields only (positions for drawing, say),
+ * instead of whole agents.  Replaces the whole-agent read of
+ * ocl_fetch_agents (tay/ocl.c:261-272) where the caller needs a few fields
+ * (the reference's viewer draws positions after every step,
+ * entorama/main.c:192-196). */
+int tay_gpu_space_fetch_words(TayGpuSpace *space, unsigned first_word, unsigned words, void *dst, unsigned count);
+
 unsigned tay_gpu_space_count(const TayGpuSpace *space);
 
 /* Rebuilds a Z-grid from the current positions: bounding box, geometry, Morton

@@ -53,3 +53,52 @@ void sph_integrate(global SphParticle *a, constant SphContext *c) {
     a->pressure_accum = float4_null();
     a->viscosity_accum = float4_null();
 }
+
+void sph_particle_density(global SphParticle *a, global SphParticle *b, constant SphContext *c) {
+    sph_density(a, b, c);
+}
+
+void sph_force_terms(global SphParticle *a, global SphParticle *b, constant SphContext *c) {
+    sph_forces(a, b, c);
+}
+
+void sph_particle_pressure(global SphParticle *a, constant SphContext *c) {
+    double ratio = (double)(a->density / c->density);
+    a->pressure = c->K * ((float)pow(ratio, 7.0) - 1.0f);
+}
+
+/* one axis of the walls: a coordinate below lo, then one above hi, is put
+ * on the wall and both velocities of the axis are scaled by damp (axes are
+ * independent, so doing them one after another keeps every result) */
+static void sph_wall(global float *p, global float *v, global float *vh, float lo, float hi, float damp) {
+    if (*p < lo) { *p = lo; *v = damp * *v; *vh = damp * *vh; }
+    if (*p > hi) { *p = hi; *v = damp * *v; *vh = damp * *vh; }
+}
+
+void sph_particle_leapfrog(global SphParticle *a, constant SphContext *c) {
+    float4 f = float4_add(a->pressure_accum, float4_mul_scalar(a->viscosity_accum, c->dynamic_viscosity));
+    float4 acc = float4_div_scalar(f, a->density);
+    acc.z -= 9.81f;
+
+    float4 vh = float4_add(a->vh, float4_mul_scalar(acc, c->dt));
+    a->vh = vh;
+    a->v = float4_add(vh, float4_mul_scalar(acc, c->dt * 0.5f));
+    a->p = float4_add(a->p, float4_mul_scalar(vh, c->dt));
+
+    const float damp = -0.5f;
+    sph_wall(&a->p.x, &a->v.x, &a->vh.x, c->min.x, c->max.x, damp);
+    sph_wall(&a->p.y, &a->v.y, &a->vh.y, c->min.y, c->max.y, damp);
+    sph_wall(&a->p.z, &a->v.z, &a->vh.z, c->min.z, c->max.z, damp);
+
+    sph_particle_reset(a);
+}
+
+void sph_particle_reset(global SphParticle *a) {
+    a->density = 0.0f;
+    a->pressure_accum.x = 0.0f;
+    a->pressure_accum.y = 0.0f;
+    a->pressure_accum.z = 0.0f;
+    a->viscosity_accum.x = 0.0f;
+    a->viscosity_accum.y = 0.0f;
+    a->viscosity_accum.z = 0.0f;
+}

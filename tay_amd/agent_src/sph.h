@@ -50,3 +50,18 @@ void sph_density(global SphParticle *a, global SphParticle *b, constant SphConte
 void sph_pressure(global SphParticle *a, constant SphContext *c);
 void sph_forces(global SphParticle *a, global SphParticle *b, constant SphContext *c);
 void sph_integrate(global SphParticle *a, constant SphContext *c);
+
+/* The reference plug-in's own pass functions, under its names
+ * (m_sph/agent.h:34-38, m_sph/main.c:70-74):
+ *   sph_particle_density   = sph_density
+ *   sph_force_terms        = sph_forces
+ *   sph_particle_pressure  Tait equation with the double-precision pow of
+ *                          the density ratio, rounded to float
+ *   sph_particle_leapfrog  leapfrog: vh += a dt, v = vh + a dt/2, p += vh dt,
+ *                          walls as sph_integrate, then sph_particle_reset
+ *   sph_particle_reset     clears density and the xyz of both accumulators */
+void sph_particle_density(global SphParticle *a, global SphParticle *b, constant SphContext *c);
+void sph_particle_pressure(global SphParticle *a, constant SphContext *c);
+void sph_force_terms(global SphParticle *a, global SphParticle *b, constant SphContext *c);
+void sph_particle_leapfrog(global SphParticle *a, constant SphContext *c);
+void sph_particle_reset(global SphParticle *a);

@@ -52,6 +52,20 @@ void box_agent_act(global BoxAgent *a, constant ActContext *c) {
     if (a->max.z > c->max.z) { a->max.z = c->max.z; a->min.z = c->max.z - size.z; a->v.z = -a->v.z; }
 }
 
+void point_box_see(global Agent *a, global BoxAgent *b, constant SeeContext *c) {
+    a->b_buffer.x += b->min.x - a->p.x;
+    a->b_buffer.y += b->min.y - a->p.y;
+    a->b_buffer.z += b->min.z - a->p.z;
+    a->b_buffer_count += 1;
+}
+
+void box_point_see(global BoxAgent *a, global Agent *b, constant SeeContext *c) {
+    a->b_buffer.x += b->p.x - a->min.x;
+    a->b_buffer.y += b->p.y - a->min.y;
+    a->b_buffer.z += b->p.z - a->min.z;
+    a->b_buffer_count += 1;
+}
+
 void plain_see(global PlainAgent *a, global PlainAgent *b, constant PlainSeeContext *c) {
     float dx = b->p.x - a->p.x;
     float dy = b->p.y - a->p.y;

@@ -10,6 +10,12 @@
  * BoxAgent / box_agent_see / box_agent_act: the non-point variant
  * (test/agent.h:19-27, test/agent.c:57-119) with min/max corners.
  *
+ * point_box_see / box_point_see: a point group seeing a box group and the
+ * reverse, which the reference's harness has no model for
+ * (test/test_combo.c:11-17): the seer's b_buffer gains the seen min corner
+ * (or position) minus its own position (or min corner), b_buffer_count
+ * counts the pairs; the act passes are agent_act and box_agent_act.
+ *
  * PlainAgent / plain_see / plain_act: the plain 3D point workload: count
  * the neighbours within Euclidean distance r (d^2 <= r^2, self excluded by
  * the pass) and sum their positions; the act pass keeps the finished step's
@@ -65,5 +71,7 @@ void agent_see(global Agent *a, global Agent *b, constant SeeContext *c);
 void agent_act(global Agent *a, constant ActContext *c);
 void box_agent_see(global BoxAgent *a, global BoxAgent *b, constant SeeContext *c);
 void box_agent_act(global BoxAgent *a, constant ActContext *c);
+void point_box_see(global Agent *a, global BoxAgent *b, constant SeeContext *c);
+void box_point_see(global BoxAgent *a, global Agent *b, constant SeeContext *c);
 void plain_see(global PlainAgent *a, global PlainAgent *b, constant PlainSeeContext *c);
 void plain_act(global PlainAgent *a, constant ActContext *c);

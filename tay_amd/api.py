@@ -114,6 +114,7 @@ _GPU = {
     "tay_set_order_bits": (_i, [_vp, _vp, _i]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),
+    "tay_fetch_agent_fields": (_u, [_vp, _vp, _u, _u, _vp]),
     "tay_gpu_available": (_i, []),
     "tay_gpu_last_error": (C.c_char_p, []),
     "tay_gpu_synchronize": (_i, []),
@@ -128,6 +129,7 @@ _GPU = {
     "tay_gpu_space_set_order_bits": (_i, [_vp, _i]),
     "tay_gpu_space_push": (_i, [_vp, _vp, _u]),
     "tay_gpu_space_fetch": (_i, [_vp, _vp, _u]),
+    "tay_gpu_space_fetch_words": (_i, [_vp, _u, _u, _vp, _u]),
     "tay_gpu_space_count": (_u, [_vp]),
     "tay_gpu_space_sort": (_i, [_vp]),
     "tay_gpu_space_grid_info": (_i, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),

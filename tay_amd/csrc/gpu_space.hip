@@ -321,6 +321,30 @@ TAY_API int tay_gpu_space_fetch(TayGpuSpace *sp, void *agents, unsigned count) {
            hip_ok(hipStreamSynchronize(s), "fetch agents");
 }
 
+TAY_API int tay_gpu_space_fetch_words(TayGpuSpace *sp, unsigned first_word, unsigned words, void *dst,
+                                     unsigned count) {
+    if (first_word > sp->words || words > sp->words - first_word) {
+        set_error("words [%u, %u + %u) outside the %u words of an agent", first_word, first_word, words, sp->words);
+        return 0;
+    }
+    hipStream_t s = stream();
+    if (!s)
+        return 0;
+    if (count > sp->count)
+        count = sp->count;
+    if (count == 0 || words == 0)
+        return 1;
+    const size_t bytes = (size_t)count * words * 4u;
+    if (!ensure_staging(sp, bytes))
+        return 0;
+    k_soa_to_aos<<<blocks_for((unsigned long long)count * words, kBuildBlock, 8192), kBuildBlock, 0, s>>>(
+        sp->cols[sp->cur] + (size_t)first_word * sp->capacity, sp->staging, count, words, sp->capacity);
+    if (!launch_ok("soa_to_aos"))
+        return 0;
+    return hip_ok(hipMemcpyAsync(dst, sp->staging, bytes, hipMemcpyDeviceToHost, s), "fetch words") &&
+           hip_ok(hipStreamSynchronize(s), "fetch words");
+}
+
 TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
     if (sp->kind != TAY_GPU_KIND_Z_GRID || sp->count == 0)
         return 1;

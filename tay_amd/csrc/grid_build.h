@@ -73,10 +73,14 @@ __device__ __forceinline__ unsigned morton(int x, int y, int z, int dims) {
 
 /* Cell part of a storage-order key: the key interleaves (cell << s) | sub on
  * every axis, so the cell's Morton key is the key without its low dims x s
- * bits. */
+ * bits.  Every key is shifted, 0xFFFFFFFF included: a 2D grid of 8192 cells
+ * per axis with 3 sub-cell bits gives that key to the agent in the top
+ * corner sub-cell, and a batch's padding key (also 0xFFFFFFFF) never reaches
+ * this function (its callers skip padding slots). */
 __device__ __forceinline__ unsigned cell_of_key(unsigned key, int dims, const TayGridGeom *g) {
-    return key == 0xFFFFFFFFu ? key : key >> ((unsigned)dims * (unsigned)g->order_bits);
+    return key >> ((unsigned)dims * (unsigned)g->order_bits);
 }
+
 
 /* Row-major cell id (x fastest) of a cell's Morton key. */
 __device__ __forceinline__ unsigned long long morton_to_cell(unsigned key, int dims, const TayGridGeom *g) {

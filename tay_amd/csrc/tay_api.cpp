@@ -112,6 +112,25 @@ int gpu_kind(const TayState *state, const TayGroup *g) {
     }
 }
 
+/* Under TAY_FETCH_ON_DEMAND the device holds the newest agents of every
+ * group whose host storage has not been changed since the last push; copy
+ * them back before the device spaces go away (simulation end, recompile,
+ * switch to host), so nothing computed is lost.  Groups with a pending push
+ * keep their host records, which the caller changed. */
+int fetch_before_release(TayState *state) {
+    if (state->fetch_mode != TAY_FETCH_ON_DEMAND)
+        return 1;
+    for (TayGroup &g : state->groups) {
+        if (!g.active || !g.gpu || g.push_agents)
+            continue;
+        if (!tay_gpu_space_fetch(g.gpu, g.storage, g.count)) {
+            set_device_error(state, "fetch agents");
+            return 0;
+        }
+    }
+    return 1;
+}
+
 void release_device(TayState *state) {
     tay_gpu_program_destroy(state->program);
     state->program = nullptr;
@@ -119,6 +138,24 @@ void release_device(TayState *state) {
         tay_gpu_space_destroy(g.gpu);
         g.gpu = nullptr;
     }
+}
+
+/* Moves a group's host storage to page-locked memory once the device path
+ * is chosen, so pushes and fetches copy at the link's full rate; the whole
+ * capacity is copied (records the caller wrote but has not committed yet
+ * included).  Storage stays ordinary memory when no page-locked memory can
+ * be had. */
+void pin_storage(TayGroup &g) {
+    if (g.storage_pinned)
+        return;
+    const size_t bytes = (size_t)(g.capacity ? g.capacity : 1) * (size_t)(g.agent_size ? g.agent_size : 1);
+    char *pinned = (char *)tay_gpu_host_alloc(bytes);
+    if (!pinned)
+        return;
+    std::memcpy(pinned, g.storage, bytes);
+    std::free(g.storage);
+    g.storage = pinned;
+    g.storage_pinned = 1;
 }
 
 /* tay/state_compile.c:5-175 for the device path: validate groups and
@@ -185,10 +222,13 @@ int state_compile(TayState *state) {
     }
     state->device_enabled = 1;
 
+    if (!fetch_before_release(state))
+        return 0;
     release_device(state);
     for (TayGroup &g : state->groups) {
         if (!g.active)
             continue;
+        pin_storage(g);
         float part[4] = {g.min_part_sizes.x, g.min_part_sizes.y, g.min_part_sizes.z, g.min_part_sizes.w};
         g.gpu = tay_gpu_space_create(gpu_kind(state, &g), g.agent_size, g.capacity, g.is_point, g.dims, part,
                                      g.shared_bytes);
@@ -289,6 +329,7 @@ TAY_API int tay_switch_to_ocl(TayState *state) {
 
 TAY_API int tay_switch_to_host(TayState *state) {
     if (state->device_enabled) {
+        fetch_before_release(state);
         release_device(state);
         state->device_enabled = 0;
         state->recompile = 1;
@@ -316,13 +357,11 @@ TAY_API TayGroup *tay_add_group(TayState *state, unsigned agent_size, unsigned a
     }
     TayGroup *g = state->groups + gi;
     std::memset(g, 0, sizeof(*g));
-    /* page-locked when a GPU is present, so fetches and pushes copy at the
-     * link's full rate; ordinary zeroed memory otherwise */
-    const size_t bytes = (size_t)(agent_capacity ? agent_capacity : 1) * (size_t)(agent_size ? agent_size : 1);
-    g->storage = (char *)tay_gpu_host_alloc(bytes);
-    g->storage_pinned = g->storage != nullptr;
-    if (!g->storage)
-        g->storage = (char *)std::calloc(agent_capacity ? agent_capacity : 1, agent_size ? agent_size : 1);
+    /* ordinary zeroed memory, as the reference's calloc; it moves to
+     * page-locked memory when the state compiles for the device
+     * (pin_storage), so creating a group touches no GPU */
+    g->storage = (char *)std::calloc(agent_capacity ? agent_capacity : 1, agent_size ? agent_size : 1);
+    g->storage_pinned = 0;
     if (!g->storage) {
         char msg[96];
         std::snprintf(msg, sizeof(msg), "cannot allocate host storage for %u agents of %u bytes", agent_capacity,
@@ -459,6 +498,7 @@ TAY_API void tay_simulation_end(TayState *state) {
         return;
     }
     state->is_running = 0;
+    fetch_before_release(state);
     release_device(state);
     state->recompile = 1;
 }
@@ -661,4 +701,22 @@ TAY_API int tay_fetch_agents(TayState *state, TayGroup *group) {
         }
     }
     return 1;
+}
+
+TAY_API unsigned tay_fetch_agent_fields(TayState *state, TayGroup *group, unsigned offset, unsigned size,
+                                        void *dst) {
+    if (offset % 4u || size % 4u || offset > group->agent_size || size > group->agent_size - offset) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "fields must be whole words within the agent");
+        return 0;
+    }
+    if (!group->gpu) {
+        set_error(state, TAY_ERROR_STATE_STATUS, "group is not on the device: run the simulation first");
+        return 0;
+    }
+    const unsigned n = tay_gpu_space_count(group->gpu);
+    if (!tay_gpu_space_fetch_words(group->gpu, offset / 4u, size / 4u, dst, n)) {
+        set_device_error(state, "fetch agent fields");
+        return 0;
+    }
+    return n;
 }

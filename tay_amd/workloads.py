@@ -68,6 +68,21 @@ def test_agents(n, seed=1, box=1000.0, clump=0.0):
     return rec
 
 
+def box_agents(n, seed=1, box=1000.0, size=(1.0, 50.0)):
+    """Non-point test agents (test/agent.h BoxAgent): min corners uniform in
+    [0, box)^3, extents uniform in `size` per axis, unit-random velocities
+    (test/test.c make_randomized_direction_cluster_nonpoint with exponent 0)."""
+    g = rng(seed)
+    rec = np.zeros(n, A.BOX_AGENT)
+    lo = g.uniform(0.0, box, size=(n, 3)).astype(np.float32)
+    ext = g.uniform(size[0], size[1], size=(n, 3)).astype(np.float32)
+    rec["min"][:, :3] = lo
+    rec["max"][:, :3] = lo + ext
+    rec["v"][:, :3] = unit_vectors(n, g)
+    rec["result_index"] = np.arange(n)
+    return rec
+
+
 def flock_agents(n, seed=1, clusters=64, sigma=30.0, box=2000.0):
     """Flocking workload: Gaussian clusters, unit-random velocities."""
     g = rng(seed)

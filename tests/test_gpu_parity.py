@@ -106,7 +106,7 @@ def test_grid_geometry_bit_exact(gpu, oracle, models):
     assert np.array_equal(c1[:3], c2[:3]) and n1 == n2
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4097])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4097, 16384, 16385])
 def test_ragged_sizes(gpu, oracle, models, n):
     recs = W.plain_agents(n, seed=100 + n, box=200.0)
     ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(30.0), recs, 2)
@@ -204,6 +204,38 @@ def test_lower_dimensional_grids_bit_exact(gpu, oracle, models, dims):
     _, sa, _, sp = run_pair(gpu, oracle, models, model, recs, 1, oracle_space=api.TAY_CPU_SIMPLE)
     ga1, _, gp1, _ = run_pair(gpu, oracle, models, model, recs, 1)
     assert gp1 == sp
+
+
+def test_2d_key_of_the_top_corner_sub_cell(gpu, oracle, models):
+    """A 2D grid of 8192 cells per axis with 3 sub-cell bits: the agent in
+    the top corner sub-cell has storage key 0xFFFFFFFF (Morton of 65535,
+    65535).  Its cell is (8191, 8191) like any other key's; bit-exact with the
+    oracle, and the corner agent's pairs are all there."""
+    m = W.test_model(2.0, box=10000.0)
+    model = W.Model(m.dtype, m.passes, 1.0, 2)
+    g = W.rng(71)
+    recs = W.test_agents(3000, seed=71)
+    recs["p"][:, :2] = g.uniform(8150.0, 8191.5, size=(3000, 2)).astype(np.float32)
+    recs["p"][:, 2] = 0.0
+    recs["v"][:] = 0.0
+    recs["p"][0, :2] = 0.0          # stretches the box to [0, 8191.5]
+    recs["p"][1, :2] = 8191.5       # the top corner: q = 8191.99.., sub-cell 7 on both axes
+    recs["p"][2, :2] = 8190.5       # a neighbour of the corner agent
+    print("case: 2D top-corner key", flush=True)
+    out = []
+    for lib, funcs in ((gpu, None), (oracle, models)):
+        with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, funcs=funcs, shared_mb=512) as s:
+            assert lib.tay_set_order_bits(s.state, s.group, 3)
+            s.run(1)
+            a = s.agents()
+            _, _, counts, _ = lib.grid_info(s.state, s.group)
+            out.append(a)
+            assert counts[0] == 8192 and counts[1] == 8192
+    assert_bitwise(out[0], out[1])
+    corner = out[0][out[0]["result_index"] == 1][0]
+    assert corner["part_i"] == 0xFFFFFFFF >> 6
+    assert corner["b_buffer_count"] == 0    # the act pass cleared it; the sum moved into f_buffer
+    assert np.all(corner["f_buffer"][:2] < 0)
 
 
 def test_unequal_axis_radii_bit_exact(gpu, oracle, models):
