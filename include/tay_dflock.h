@@ -61,6 +61,11 @@ int tay_dflock_times(TayDflock *flock, double *forward_ms, double *backward_ms);
 /* Largest neighbour count of any agent in any step of the last call. */
 int tay_dflock_max_neighbours(TayDflock *flock, unsigned *count);
 
+/* Neighbour pairs (r^2 < R^2, each agent's own count, self excluded) summed
+ * over the steps of the last forward call: the pair interactions of the
+ * forward pass. */
+int tay_dflock_pair_count(TayDflock *flock, unsigned long long *pairs);
+
 #ifdef __cplusplus
 }
 #endif

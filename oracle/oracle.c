@@ -104,6 +104,7 @@ typedef struct {
     uint64_t *sort_buf;     /* (key << 32 | index) pairs */
     int order_bits;         /* requested sub-cell bits of the storage order */
     int gz_order_bits;      /* the ones used by the last sort */
+    float gz_max_extent[4]; /* largest max - min corner per axis (non-point agents) */
 } Space;
 
 typedef struct TayGroup {
@@ -135,6 +136,7 @@ typedef struct TayPass {
     char func_name[MAX_FUNC_NAME];
     PairingFunc pairing;
     PairCounts counts;      /* summed over the steps of the last run */
+    unsigned seer_limit;    /* see passes: only the first seer_limit seers (0: all) */
 } TayPass;
 
 typedef struct TayState {
@@ -1014,6 +1016,19 @@ static void gpu_z_grid_sort(TayGroup *g) {
     Space *s = &g->space;
     update_box(g);
     gpu_z_grid_geometry(s);
+    /* non-point agents are binned by their min corner; a seer's seen range
+     * then reaches down by the largest extent of the group */
+    for (int k = 0; k < 4; ++k)
+        s->gz_max_extent[k] = 0.0f;
+    if (!g->is_point)
+        for (unsigned a = 0; a < s->count; ++a) {
+            const char *agent = g->storage + (size_t)g->agent_size * a;
+            for (int k = 0; k < s->dims; ++k) {
+                float e = MAXC(agent)[k] - POS(agent)[k];
+                if (e > s->gz_max_extent[k])
+                    s->gz_max_extent[k] = e;
+            }
+        }
 
     /* Storage-order key: per axis q = (p - o) / cs, the cell c = floorf(q)
      * (clamped: only a fixed box can leave agents outside) refined by
@@ -1075,13 +1090,17 @@ static void visit_gpu_z_cell(TayPass *p, Slice seer, Int4 idx, PairCounts *c) {
     p->pairing(p, seer, seen_slice(p, r->beg, r->end), c);
 }
 
-/* per-seer conservative cell range */
+/* Per-seer conservative cell range: the seen min corners (positions) that
+ * can pass the box test lie in [lo - r - extent, hi + r], extent the seen
+ * group's largest box extent (0 for points), widened by a margin of
+ * (|p| + r + extent) * 2^-20 for the rounding of these sums and of the test. */
 static void gpu_z_grid_range(const Space *s, const float *lo_pos, const float *hi_pos, const float *r, int dims, Int4 *lo, Int4 *hi) {
     Box box;
     for (int i = 0; i < dims; ++i) {
-        float mlo = (fabsf(lo_pos[i]) + r[i]) * 0x1p-20f;
+        float ext = s->gz_max_extent[i];
+        float mlo = (fabsf(lo_pos[i]) + r[i] + ext) * 0x1p-20f;
         float mhi = (fabsf(hi_pos[i]) + r[i]) * 0x1p-20f;
-        box.min.arr[i] = (lo_pos[i] - r[i]) - mlo;
+        box.min.arr[i] = ((lo_pos[i] - r[i]) - ext) - mlo;
         box.max.arr[i] = (hi_pos[i] + r[i]) + mhi;
     }
     clamp_range(s, box, dims, lo, hi);
@@ -1134,7 +1153,8 @@ static void see_task(void *task_p, int thread_i, PairCounts *c) {
     const Space *ss = &seer->space;
     int dims = ss->dims < p->seen_group->space.dims ? ss->dims : p->seen_group->space.dims;
     unsigned beg, end;
-    thread_range(ss->count, thread_i, &beg, &end);
+    unsigned seers = p->seer_limit && p->seer_limit < ss->count ? p->seer_limit : ss->count;
+    thread_range(seers, thread_i, &beg, &end);
 
     if (ss->type != TAY_CPU_GRID && ss->type != TAY_CPU_Z_GRID) {
         TaySpaceType seen_type = p->seen_group->space.type;
@@ -1225,7 +1245,9 @@ static int state_compile(TayState *state) {
             set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "space type not restated in the oracle");
             return 0;
         }
-        if (!g->is_point && is_grid(t)) {
+        /* the reference's grids take points only (tay/state_compile.c:25-27);
+         * the GPU Z-grid also bins boxes, by their min corner */
+        if (!g->is_point && is_grid(t) && t != TAY_GPU_Z_GRID) {
             set_error(state, TAY_ERROR_POINT_NONPOINT_MISMATCH, "grid structure can only contain point agents");
             return 0;
         }
@@ -1360,4 +1382,9 @@ void oracle_get_grid_info(TayState *state, TayGroup *group, float4 *origin, floa
 
 double oracle_get_see_seconds(TayState *state) {
     return state->see_seconds;
+}
+
+void oracle_limit_seers(TayState *state, unsigned pass_index, unsigned count) {
+    if (pass_index < state->passes_count)
+        state->passes[pass_index].seer_limit = count;
 }

@@ -49,6 +49,11 @@ int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
 /* Wall-clock seconds spent in see passes during the last tay_run. */
 double oracle_get_see_seconds(TayState *state);
 
+/* A see pass that runs only its first `count` seers (in storage order after
+ * the sort; 0: all), so a single-thread baseline can time a bounded sample of
+ * a large step: pairs and see seconds then cover that sample. */
+void oracle_limit_seers(TayState *state, unsigned pass_index, unsigned count);
+
 #ifdef __cplusplus
 }
 #endif

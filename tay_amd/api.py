@@ -171,6 +171,7 @@ _GPU = {
     "tay_dflock_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "tay_dflock_times": (_i, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "tay_dflock_max_neighbours": (_i, [_vp, C.POINTER(C.c_uint)]),
+    "tay_dflock_pair_count": (_i, [_vp, C.POINTER(C.c_ulonglong)]),
     "tay_gpu_batch_autotune": (_i, [_vp, C.POINTER(C.c_float), _u, C.POINTER(C.c_uint), _u, C.POINTER(C.c_uint), _u,
                                     _u, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint),
                                     C.POINTER(C.c_uint)]),
@@ -184,6 +185,7 @@ _ORACLE = {
     "oracle_morton_key": (_u, [_i, _i, _i, _i]),
     "oracle_morton_decode": (None, [_u, _i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "oracle_get_see_seconds": (C.c_double, [_vp]),
+    "oracle_limit_seers": (None, [_vp, _u, _u]),
 }
 
 

@@ -1,6 +1,7 @@
 /*
  * device_abi.h - structures shared by the ahead-of-time grid kernels
- * (grid_kernels.hip), the host code and the kernels generated at run time.
+ * (grid_build.h, used by gpu_space.hip, gpu_batch.hip and dflock.hip), the
+ * host code and the kernels generated at run time.
  * The build embeds this file's text at the top of every hipRTC program, so it
  * must compile both under hipcc and under hipRTC (no includes).
  */
@@ -17,6 +18,11 @@ struct TayGridGeom {
     float box_min[4];
     float box_max[4];
     int order_bits;             /* sub-cell bits per axis of the storage-order key */
+    /* largest extent (max corner - min corner) of the group's agents per
+     * axis; 0 for point agents.  Non-point agents are binned by their min
+     * corner, so a search for agents whose box reaches a region starts this
+     * far below it. */
+    float max_extent[4];
 };
 
 /* One cell's agents are [beg, end) of the key-sorted storage. */

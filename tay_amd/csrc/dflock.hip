@@ -123,7 +123,8 @@ __device__ __forceinline__ unsigned group_exclusive_count(unsigned v, unsigned s
  * lane l is word (t * KL + e) * LANES + l, so the lanes of an agent that
  * read their e-th entries together read one contiguous run.  max_count[0] takes the largest
  * neighbour count of an agent, max_count[1] the largest of one lane, which
- * sizes the lists of the recording recomputation. */
+ * sizes the lists of the recording recomputation; pair_total (when given)
+ * adds up every agent's neighbour count. */
 template <bool RECORD>
 __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols, unsigned n,
                                                  const unsigned *__restrict__ ids, const float *__restrict__ srt,
@@ -131,7 +132,8 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
                                                  const TayGridGeom *__restrict__ g, Prm c,
                                                  float *__restrict__ out, unsigned *__restrict__ nbr, unsigned KL,
                                                  unsigned *__restrict__ counts, float *__restrict__ sums,
-                                                 unsigned *__restrict__ max_count) {
+                                                 unsigned *__restrict__ max_count,
+                                                 unsigned long long *__restrict__ pair_total) {
     const unsigned t = (blockIdx.x * 256u + threadIdx.x) / LANES;
     const unsigned sub = threadIdx.x % LANES;
     const bool live = t < n;
@@ -211,6 +213,14 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     if ((threadIdx.x & 63u) == 0u && m) {
         atomicMax(max_count, m);
         atomicMax(max_count + 1, ml);
+    }
+    if (pair_total) {
+        /* each agent's count once (its lanes hold partial counts) */
+        unsigned long long own = live ? cnt : 0u;
+        for (int o = 32; o > 0; o >>= 1)
+            own += __shfl_xor(own, o);
+        if ((threadIdx.x & 63u) == 0u && own)
+            atomicAdd(pair_total, own);
     }
 }
 
@@ -563,7 +573,7 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->cols[1], state_bytes(f), "agents") &&
               hip_ok(hipMemsetAsync(f->cols[0], 0, state_bytes(f), stream()), "hipMemsetAsync") &&
               hip_ok(hipMemsetAsync(f->cols[1], 0, state_bytes(f), stream()), "hipMemsetAsync") &&
-              device_alloc((void **)&f->box_partials, sizeof(float) * 6 * f->box_blocks, "box partials") &&
+              device_alloc((void **)&f->box_partials, sizeof(float) * kBoxPartial * f->box_blocks, "box partials") &&
               device_alloc((void **)&f->geom, sizeof(TayGridGeom), "geometry") &&
               device_alloc((void **)&f->keys[0], 4ull * n, "keys") && device_alloc((void **)&f->keys[1], 4ull * n, "keys") &&
               device_alloc((void **)&f->vals[0], 4ull * n, "agents by key") &&
@@ -571,7 +581,7 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->block_hist, 4ull * kRadix * nblocks, "radix histogram") &&
               device_alloc((void **)&f->scan_sums, 4ull * kScanBlocks, "scan sums") &&
               device_alloc((void **)&f->ranges, sizeof(TayCellRange) * max_cells, "cell table") &&
-              device_alloc((void **)&f->max_count, 8, "neighbour maxima") &&
+              device_alloc((void **)&f->max_count, 16, "neighbour maxima and pair total") &&
               device_alloc((void **)&f->seg_states, state_bytes(f) * checkpoint_every, "segment states") &&
               device_alloc((void **)&f->seg_ids, 4ull * n * checkpoint_every, "segment orders") &&
               device_alloc((void **)&f->seg_sums, 10ull * n * 4 * checkpoint_every, "segment sums") &&
@@ -616,7 +626,7 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
     ok = ok && hip_ok(hipMemcpyAsync(f->io, v0, rows3, hipMemcpyHostToDevice, s), "upload v0");
     if (ok)
         k_df_unpack<<<blocks, 256, 0, s>>>(f->io, n, f->cols[0] + 5ull * n);
-    ok = ok && hip_ok(hipMemsetAsync(f->max_count, 0, 8, s), "hipMemsetAsync") &&
+    ok = ok && hip_ok(hipMemsetAsync(f->max_count, 0, 16, s), "hipMemsetAsync") &&
          hip_ok(hipStreamSynchronize(s), "upload initial state");
     if (!ok)
         return 0;
@@ -632,7 +642,8 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
         df_sort(f, f->cols[cur], s);
         k_df_sorted_copy<<<blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->srt);
         k_df_step<false><<<lane_blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->srt, f->ranges, f->geom, c,
-                                                f->cols[cur ^ 1], nullptr, 0, nullptr, nullptr, f->max_count);
+                                                f->cols[cur ^ 1], nullptr, 0, nullptr, nullptr, f->max_count,
+                                                (unsigned long long *)(f->max_count + 2));
         cur ^= 1;
     }
     ok = ok && launch_ok("differentiable forward") && timed_end(ev, s, &f->forward_ms);
@@ -718,7 +729,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], srt, f->ranges, f->geom, c, next,
                                                    f->seg_nbr + (size_t)r * KL * LANES * n, KL,
                                                    f->seg_counts + (size_t)r * LANES * n,
-                                                   f->seg_sums + (size_t)r * 10 * n, f->max_count);
+                                                   f->seg_sums + (size_t)r * 10 * n, f->max_count, nullptr);
         }
         /* reverse the segment's steps */
         for (unsigned t = t1; ok && t-- > t0;) {
@@ -773,4 +784,10 @@ TAY_API int tay_dflock_max_neighbours(TayDflock *f, unsigned *count) {
     hipStream_t s = stream();
     return s && hip_ok(hipMemcpyAsync(count, f->max_count, 4, hipMemcpyDeviceToHost, s), "read neighbour maximum") &&
            hip_ok(hipStreamSynchronize(s), "read neighbour maximum");
+}
+
+TAY_API int tay_dflock_pair_count(TayDflock *f, unsigned long long *pairs) {
+    hipStream_t s = stream();
+    return s && hip_ok(hipMemcpyAsync(pairs, f->max_count + 2, 8, hipMemcpyDeviceToHost, s), "read pair total") &&
+           hip_ok(hipStreamSynchronize(s), "read pair total");
 }

@@ -175,10 +175,6 @@ TAY_API TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigne
         set_error("%d dimensions not supported by this space (Z-grid: 1 to 3)", dims);
         return nullptr;
     }
-    if (kind == TAY_GPU_KIND_Z_GRID && !is_point) {
-        set_error("grid structure can only contain point agents");
-        return nullptr;
-    }
     /* kernels index a column word as word * capacity + slot, and a slot's
      * byte offset in a column, in 32 bits */
     if ((unsigned long long)(agent_size / 4) * (capacity ? capacity : 1) > 0xFFFFFFFFull) {
@@ -222,7 +218,7 @@ TAY_API TayGpuSpace *tay_gpu_space_create(int kind, unsigned agent_size, unsigne
         sp->box_blocks = blocks_for(sp->capacity, kBuildBlock * 4, 1024);
         sp->radix_blocks = (sp->capacity + kRadixTile - 1) / kRadixTile;
         ok = ok && device_alloc((void **)&sp->geom, sizeof(TayGridGeom), "grid geometry") &&
-             device_alloc((void **)&sp->box_partials, sizeof(float) * 6 * sp->box_blocks, "box partials") &&
+             device_alloc((void **)&sp->box_partials, sizeof(float) * kBoxPartial * sp->box_blocks, "box partials") &&
              device_alloc((void **)&sp->keys[0], 4ull * sp->capacity, "sort keys") &&
              device_alloc((void **)&sp->keys[1], 4ull * sp->capacity, "sort keys") &&
              device_alloc((void **)&sp->vals[0], 4ull * sp->capacity, "sort values") &&

@@ -81,7 +81,6 @@ __device__ __forceinline__ unsigned cell_of_key(unsigned key, int dims, const Ta
     return key >> ((unsigned)dims * (unsigned)g->order_bits);
 }
 
-
 /* Row-major cell id (x fastest) of a cell's Morton key. */
 __device__ __forceinline__ unsigned long long morton_to_cell(unsigned key, int dims, const TayGridGeom *g) {
     unsigned x, y = 0, z = 0;
@@ -117,12 +116,16 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 /* Per-block bounding box of the positions (min corner for every agent; for
- * non-point agents the max corner at words 6..8). */
+ * non-point agents the max corner at words 6..8) and, for non-point agents,
+ * the largest extent max - min per axis: kBoxPartial floats per block. */
+constexpr unsigned kBoxPartial = 9;
+
 __global__ __launch_bounds__(kBuildBlock) void k_box_partial(const unsigned *__restrict__ cols, unsigned cap,
                                                              unsigned n, int dims, int is_point,
                                                              float *__restrict__ partials) {
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
     float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    float ext[3] = {0.0f, 0.0f, 0.0f};
     for (unsigned i = blockIdx.x * kBuildBlock + threadIdx.x; i < n; i += gridDim.x * kBuildBlock) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -131,26 +134,29 @@ __global__ __launch_bounds__(kBuildBlock) void k_box_partial(const unsigned *__r
                 float q = is_point ? p : __uint_as_float(cols[(6u + k) * cap + i]);
                 lo[k] = fminf(lo[k], p);
                 hi[k] = fmaxf(hi[k], q);
+                if (!is_point)
+                    ext[k] = fmaxf(ext[k], q - p);
             }
         }
     }
-    __shared__ float s[kBuildBlock / 64][6];
+    __shared__ float s[kBuildBlock / 64][kBoxPartial];
     const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float a = wave_min(lo[k]), b = wave_max(hi[k]);
+        float a = wave_min(lo[k]), b = wave_max(hi[k]), e = wave_max(ext[k]);
         if (lane == 0) {
             s[w][k] = a;
             s[w][3 + k] = b;
+            s[w][6 + k] = e;
         }
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < kBoxPartial) {
         const unsigned k = threadIdx.x;
         float v = s[0][k];
         for (unsigned j = 1; j < kBuildBlock / 64; ++j)
             v = k < 3 ? fminf(v, s[j][k]) : fmaxf(v, s[j][k]);
-        partials[blockIdx.x * 6u + k] = v;
+        partials[blockIdx.x * kBoxPartial + k] = v;
     }
 }
 
@@ -209,6 +215,7 @@ __device__ void grid_geometry(const float *lo, const float *hi, const float *par
             geom->box_min[a] = 0.0f;
             geom->box_max[a] = 0.0f;
         }
+        geom->max_extent[a] = 0.0f;
     }
     geom->cells = cells;
     /* the finest sub-cell order whose key still fits the Morton range */
@@ -223,7 +230,8 @@ __device__ void grid_geometry(const float *lo, const float *hi, const float *par
     geom->order_bits = s;
 }
 
-/* One wavefront: reduce the block boxes and derive the grid. */
+/* One wavefront: reduce the block boxes and derive the grid; the largest
+ * extent of non-point agents goes into the geometry. */
 __global__ __launch_bounds__(64) void k_geometry(const float *__restrict__ partials, unsigned nparts, int dims,
                                                  float px, float py, float pz, unsigned long long max_cells,
                                                  int order_bits, int box_fixed, float fx0, float fy0, float fz0,
@@ -231,17 +239,20 @@ __global__ __launch_bounds__(64) void k_geometry(const float *__restrict__ parti
     const unsigned lane = threadIdx.x;
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
     float hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    float ext[3] = {0.0f, 0.0f, 0.0f};
     for (unsigned b = lane; b < nparts; b += 64) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], partials[b * 6u + k]);
-            hi[k] = fmaxf(hi[k], partials[b * 6u + 3 + k]);
+            lo[k] = fminf(lo[k], partials[b * kBoxPartial + k]);
+            hi[k] = fmaxf(hi[k], partials[b * kBoxPartial + 3 + k]);
+            ext[k] = fmaxf(ext[k], partials[b * kBoxPartial + 6 + k]);
         }
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         lo[k] = wave_min(lo[k]);
         hi[k] = wave_max(hi[k]);
+        ext[k] = wave_max(ext[k]);
     }
     if (lane != 0)
         return;
@@ -251,6 +262,8 @@ __global__ __launch_bounds__(64) void k_geometry(const float *__restrict__ parti
     }
     const float part[3] = {px, py, pz};
     grid_geometry(lo, hi, part, dims, max_cells, order_bits, geom);
+    for (int k = 0; k < dims; ++k)
+        geom->max_extent[k] = ext[k];
 }
 
 __device__ __forceinline__ int cell_axis(float p, float origin, float size, int count) {

@@ -393,10 +393,19 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
          * other's neighbourhoods; each seer still sees its cells in x->y->z
          * order. */
         const TayGridGeom *g = v.geom;
+        /* Box agents are binned by their min corner: a seen agent whose box
+         * reaches down to p - r has its min corner at most the group's
+         * largest extent below that. */
+        float ext[3] = {0.0f, 0.0f, 0.0f};
+        if (SEEN_BOX) {
+#pragma unroll
+            for (int k = 0; k < DIMS; ++k)
+                ext[k] = g->max_extent[k];
+        }
         unsigned limit = 1;
 #pragma unroll
         for (int k = 0; k < DIMS; ++k)
-            limit *= (unsigned)ceilf(2.0f * r[k] / g->cell_size[k]) + 3u;
+            limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / g->cell_size[k]) + 3u;
         const unsigned cx = (unsigned)g->count[0];
         const unsigned cxy = cx * (unsigned)g->count[1];
         const unsigned n_active = v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u;
@@ -409,12 +418,12 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
             for (int k = 0; k < DIMS; ++k) {
                 float tmin = tay_wave_min(member ? a_lo[k] : __builtin_inff());
                 float tmax = tay_wave_max(member ? a_hi[k] : -__builtin_inff());
-                /* margin (|p| + r) * 2^-20 covers the rounding of p - r and
-                 * of the box test, so no cell holding a passing pair is
-                 * missed */
-                float mlo = (fabsf(tmin) + r[k]) * 0x1p-20f;
+                /* margin (|p| + r + extent) * 2^-20 covers the rounding of
+                 * p - r - extent, of the extents and of the box test, so no
+                 * cell holding a passing pair is missed */
+                float mlo = (fabsf(tmin) + r[k] + ext[k]) * 0x1p-20f;
                 float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
-                lo[k] = tay_cell_axis((tmin - r[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
+                lo[k] = tay_cell_axis(((tmin - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
                 hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                 cells *= (unsigned)(hi[k] - lo[k] + 1);
             }
@@ -436,9 +445,10 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                     int plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
 #pragma unroll
                     for (int k = 0; k < DIMS; ++k) {
-                        const float mlo = (fabsf(a_lo[k]) + r[k]) * 0x1p-20f;
+                        const float mlo = (fabsf(a_lo[k]) + r[k] + ext[k]) * 0x1p-20f;
                         const float mhi = (fabsf(a_hi[k]) + r[k]) * 0x1p-20f;
-                        plo[k] = tay_cell_axis((a_lo[k] - r[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
+                        plo[k] = tay_cell_axis(((a_lo[k] - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k],
+                                               g->count[k]);
                         phi[k] = tay_cell_axis((a_hi[k] + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                     }
                     for (int x = plo[0]; x <= phi[0]; ++x)

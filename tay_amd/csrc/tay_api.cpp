@@ -174,7 +174,9 @@ int state_compile(TayState *state) {
     for (TayGroup &g : state->groups) {
         if (!g.active)
             continue;
-        const bool grid = g.type == TAY_CPU_GRID || g.type == TAY_CPU_Z_GRID || g.type == TAY_GPU_Z_GRID;
+        /* the reference's grids take points only (tay/state_compile.c:25-27);
+         * the GPU Z-grid also takes box agents, binned by their min corner */
+        const bool grid = g.type == TAY_CPU_GRID || g.type == TAY_CPU_Z_GRID;
         if (!g.is_point && grid) {
             set_error(state, TAY_ERROR_POINT_NONPOINT_MISMATCH, "grid structure can only contain point agents");
             return 0;

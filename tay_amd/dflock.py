@@ -55,6 +55,12 @@ class DiffFlock:
         self._ok(self.lib.tay_dflock_max_neighbours(self.flock, C.byref(m)))
         return m.value
 
+    def pair_count(self):
+        """Neighbour pairs summed over the steps of the last forward call."""
+        m = C.c_ulonglong()
+        self._ok(self.lib.tay_dflock_pair_count(self.flock, C.byref(m)))
+        return m.value
+
     def close(self):
         if self.flock:
             self.lib.tay_dflock_destroy(self.flock)

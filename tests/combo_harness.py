@@ -22,13 +22,14 @@ def is_point(rec):
 
 
 def run_groups(lib, groups, sees, radius, steps, funcs=None, fixed=None, switch=None, shared_mb=250,
-               every_step=False, setup=None):
+               every_step=False, setup=None, acts=True):
     """groups: list of (records, space type, partition size); sees: list of
     (seer, seen) group indices.  Returns the groups' records after the run,
     or, with every_step, a list of them after each step.  switch (default:
     when a group has a CPU space type) calls tay_switch_to_ocl first; fixed
     maps a group index to its (min, max) box; setup(lib, state, group
-    handles) runs before the simulation starts."""
+    handles) runs before the simulation starts; acts=False leaves out the
+    act passes (the see sums and counts then stay in the records)."""
     state = lib.tay_create_state()
     if switch is None:
         switch = any(space in (api.TAY_CPU_SIMPLE, api.TAY_CPU_Z_GRID) for _, space, _ in groups)
@@ -58,7 +59,8 @@ def run_groups(lib, groups, sees, radius, steps, funcs=None, fixed=None, switch=
                         float4(radius, radius, radius, radius), 0, see_ctx.ctypes.data, see_ctx.itemsize)
     for g, (rec, _, _) in zip(handles, groups):
         name = ACT[is_point(rec)]
-        lib.tay_add_act(state, g, fn(name), name.encode(), act.ctypes.data, act.itemsize)
+        if acts:
+            lib.tay_add_act(state, g, fn(name), name.encode(), act.ctypes.data, act.itemsize)
     for g, (rec, _, _) in zip(handles, groups):
         lib.add_agents(state, g, rec)
     if setup is not None:

@@ -77,6 +77,33 @@ def test_point_group_with_box_group(gpu, oracle, models, seen_point_space):
     assert got[0]["f_buffer"][:, :3].any() and got[1]["f_buffer"][:, :3].any()
 
 
+@pytest.mark.parametrize("case", ["one_group", "two_groups", "with_points"])
+def test_box_agents_in_z_grid(gpu, oracle, models, case):
+    """Non-point agents in the GPU Z-grid (binned by min corner, the seen
+    range grown by the group's largest extent): cells of 20 against extents
+    up to 50, bit-exact against the oracle's statement over three steps;
+    after one see pass every agent has met exactly the pairs a Simple space
+    gives it (the all-pairs set; tests/test_oracle_golden.py pins the
+    statement's pairs to the reference's nonpoint loop)."""
+    a = W.box_agents(3000, seed=59)
+    b = W.box_agents(2000, seed=60, size=(0.5, 5.0))
+    p = W.test_agents(2500, seed=61, clump=0.2)
+    groups = {"one_group": [(a, Z, 20.0)], "two_groups": [(a, Z, 20.0), (b, Z, 10.0)],
+              "with_points": [(p, Z, 20.0), (a, Z, 20.0)]}[case]
+    sees = [(0, 0)] if case == "one_group" else ALL_SEES
+    print("case: box agents in Z-grid, %s" % case, flush=True)
+    got = against_oracle(gpu, oracle, models, groups, sees, 20.0, 3)
+    assert all(g["f_buffer"][:, :3].any() for g in got)
+    zgrid = run_groups(gpu, groups, sees, 20.0, 1, acts=False)
+    simple = run_groups(gpu, [(rec, api.TAY_GPU_SIMPLE, part) for rec, _, part in groups], sees, 20.0, 1,
+                        acts=False)
+    for x, y in zip(zgrid, simple):
+        x = x[np.argsort(x["result_index"])]
+        y = y[np.argsort(y["result_index"])]
+        assert np.array_equal(x["b_buffer_count"], y["b_buffer_count"])
+        assert x["b_buffer_count"].sum() > len(x)
+
+
 def combo_inputs(d, prefix, box=False):
     n = len(d[prefix + ("_min" if box else "_p")])
     rec = np.zeros(n, A.BOX_AGENT if box else A.AGENT)

@@ -164,7 +164,10 @@ def test_run_before_start_is_a_state_error_or_device_error(tay):
 
 
 def test_non_point_group_in_a_grid(tay):
-    st, g = _state_with_group(tay, api.TAY_GPU_Z_GRID, is_point=0, size=A.BOX_AGENT.itemsize)
+    """The reference's grids take points only (tay/state_compile.c:25-27),
+    and so does TAY_CPU_Z_GRID here; the GPU Z-grid takes box agents too
+    (tests/test_gpu_combos.py)."""
+    st, g = _state_with_group(tay, api.TAY_CPU_Z_GRID, is_point=0, size=A.BOX_AGENT.itemsize)
     tay.tay_simulation_start(st)
     assert tay.tay_run(st, 1) == 0
     assert tay.tay_get_error(st) == api.TAY_ERROR_POINT_NONPOINT_MISMATCH

@@ -192,3 +192,41 @@ def test_gpu_z_grid_statement_with_previous_order_is_reference_cpu_grid(oracle, 
         want = np.zeros((len(a), 3), np.float32)
         want[order] = d[key][0]
         assert np.array_equal(bits(a[field][:, :3]), bits(want))
+
+
+def test_gpu_z_grid_statement_with_box_agents_finds_reference_pairs(oracle, models):
+    """Box agents binned by min corner in the GPU Z-grid statement, cells of
+    20 against extents up to 50: every agent meets exactly the pairs of the
+    reference's all-pairs nonpoint loop (per-agent counts equal the
+    reference's), and f_buffer after three steps equals it up to summation
+    order."""
+    from combo_harness import run_groups
+    d = np.load(os.path.join(GOLDEN, "ref_nonpoint.npz"))
+    rec = np.zeros(len(d["min"]), A.BOX_AGENT)
+    rec["min"], rec["max"], rec["v"] = d["min"], d["max"], d["v"]
+    rec["result_index"] = np.arange(len(rec))
+    r = float(d["radius"])
+    assert (rec["max"][:, :3] - rec["min"][:, :3]).max() > 2 * r
+    state = oracle.tay_create_state()
+    g = oracle.tay_add_group(state, A.BOX_AGENT.itemsize, len(rec), 0)
+    oracle.tay_configure_space(state, g, api.TAY_GPU_Z_GRID, 3, api.float4(r, r, r, r), 250)
+    ctx = np.zeros(1, A.SEE_CONTEXT)
+    import ctypes as C
+    oracle.tay_add_see(state, g, g, C.cast(models.box_agent_see, C.c_void_p), b"box_agent_see",
+                       api.float4(r, r, r, r), 0, ctx.ctypes.data, ctx.itemsize)
+    oracle.add_agents(state, g, rec)
+    oracle.tay_simulation_start(state)
+    assert oracle.tay_run(state, 1) == 1
+    a = oracle.get_agents(state, g, len(rec), A.BOX_AGENT)
+    oracle.tay_simulation_end(state)
+    oracle.tay_destroy_state(state)
+    a = a[np.argsort(a["result_index"])]
+    want = np.zeros(len(rec), np.int32)
+    want[d["simple_order"][0]] = d["simple_b_count"][0]
+    assert np.array_equal(a["b_buffer_count"], want) and want.sum() > 500
+
+    (out,) = run_groups(oracle, [(rec, api.TAY_GPU_Z_GRID, r)], [(0, 0)], r, int(d["steps"]), funcs=models)
+    out = out[np.argsort(out["result_index"])]
+    f_ref = np.zeros((len(out), 3), np.float32)
+    f_ref[d["simple_order"][-1]] = d["simple_f_buffer"][-1]
+    np.testing.assert_allclose(out["f_buffer"][:, :3], f_ref, rtol=0, atol=1e-3)
