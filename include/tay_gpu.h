@@ -101,6 +101,19 @@ int tay_gpu_space_fetch(TayGpuSpace *space, void *agents, unsigned count);
  * entorama/main.c:192-196). */
 int tay_gpu_space_fetch_words(TayGpuSpace *space, unsigned first_word, unsigned words, void *dst, unsigned count);
 
+/* The same copy without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight viewer loops, harnesses that forbid hipMalloc):
+ * tay_gpu_space_fetch_words allocates its staging buffer on first use and on
+ * growth, this variant stages in a fixed 32 MiB device array that each GPU
+ * holds until the process ends (count x words x 4 bytes must fit it, or the
+ * call fails with an error naming both sizes).  Every call on a GPU shares
+ * that array: calls that overlap on one GPU (from several host threads)
+ * race.  For a copy without host-side staging dst should be page-locked
+ * (tay_gpu_host_alloc). */
+int tay_gpu_space_fetch_words_static(TayGpuSpace *space, unsigned first_word, unsigned words, void *dst,
+                                     unsigned count);
+
 unsigned tay_gpu_space_count(const TayGpuSpace *space);
 
 /* Rebuilds a Z-grid from the current positions: bounding box, geometry, Morton

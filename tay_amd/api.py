@@ -130,6 +130,7 @@ _GPU = {
     "tay_gpu_space_push": (_i, [_vp, _vp, _u]),
     "tay_gpu_space_fetch": (_i, [_vp, _vp, _u]),
     "tay_gpu_space_fetch_words": (_i, [_vp, _u, _u, _vp, _u]),
+    "tay_gpu_space_fetch_words_static": (_i, [_vp, _u, _u, _vp, _u]),
     "tay_gpu_space_count": (_u, [_vp]),
     "tay_gpu_space_sort": (_i, [_vp]),
     "tay_gpu_space_grid_info": (_i, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),

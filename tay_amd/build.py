@@ -35,7 +35,7 @@ HOST_FLAGS = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["gpu_space.hip", "gpu_batch.hip", "dflock.hip", "jit.cpp", "tay_api.cpp"]
+SOURCES = ["gpu_space.hip", "gpu_batch.hip", "dflock.hip", "gpu_static.hip", "jit.cpp", "tay_api.cpp"]
 HOST_ONLY = {"tay_api.cpp"}
 EMBEDDED = {"device_abi_text": "device_abi.h", "device_math_text": "device_math.h", "jit_device_text": "jit_device.h"}
 
