@@ -280,6 +280,8 @@ def bench_flocking(lib, args):
         with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as twin:
             lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
             lib.tay_set_pair_counting(twin.state, 1)
+            if args.order_bits >= 0:
+                lib.tay_set_order_bits(twin.state, twin.group, args.order_bits)
             advance(twin, advance_steps, name + " counting twin")
             if warmup:
                 twin.run(warmup)
@@ -287,6 +289,8 @@ def bench_flocking(lib, args):
             p_see = lib.pair_counts(twin.state, 0)[1]
         with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as sim:
             lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
+            if args.order_bits >= 0:
+                lib.tay_set_order_bits(sim.state, sim.group, args.order_bits)
             advance(sim, advance_steps, name + " timed state")
             if warmup:
                 sim.run(warmup)
@@ -325,6 +329,7 @@ def bench_flocking(lib, args):
         "config": {"workload": "flocking-1M", "agents": args.agents, "clusters": 64, "sigma": 30.0,
                    "box": 2000.0, "radius": 25.0, "cell_size": args.cell_size, "weights": [1.5, 1.0, 1.0], "dt": 0.1,
                    "max_speed": 2.0, "structure": "TAY_GPU_Z_GRID",
+                   **({"order_bits": args.order_bits} if args.order_bits >= 0 else {}),
                    **({"advanced_steps": args.advance} if args.advance else {})},
         "steps_per_s": args.steps / wall,
         "steps_per_s_resident": args.steps / (wall - fetch),
@@ -691,6 +696,8 @@ def main():
                          "window of the trajectory; not part of the default line)")
     ap.add_argument("--agents", type=int, default=1_000_000)
     ap.add_argument("--cell-size", type=float, default=25.0, help="Z-grid cell size (min_part_sizes)")
+    ap.add_argument("--order-bits", type=int, default=-1,
+                    help="sub-cell bits of the storage order inside a cell (0-3; default: the library's)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-single-sample", type=int, default=100_000,

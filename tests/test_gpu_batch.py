@@ -143,3 +143,29 @@ def test_batch_rejects_bad_arguments(gpu):
         assert b"tile width" in gpu.tay_gpu_last_error()
         assert not gpu.tay_gpu_batch_set_radii(b.batch, 1, None)
         assert b"not a see pass" in gpu.tay_gpu_last_error()
+
+
+def test_full_size_ensemble_matches_per_instance_states_and_oracle(gpu, oracle, models):
+    """The benchmark's ensemble at its real shape: 256 instances of 1k-64k
+    agents (3.9M in all), box sides 200-1500, radii 10-40, cell factor 1.5;
+    two graph-replayed steps, every instance bit for bit equal to its own
+    TAY_GPU_Z_GRID state, and four instances (the smallest, the largest and
+    two seeded others) equal to the oracle's."""
+    ens = W.ensemble(256, seed=7)
+    part, tables = parts_and_tables(ens, 1.5)
+    print("case: full-size ensemble, 256 instances, 2 steps, graph", flush=True)
+    with W.Batch(gpu, ens, part, tables, count_pairs=True) as b:
+        b.run(2, graph=True)
+        got = b.agents()
+        bpairs = b.pair_count(0)
+    want, pairs = per_instance(gpu, ens, part, tables, 2)
+    for g, w in zip(got, want):
+        assert_bitwise(g, w)
+    assert bpairs == sum(pairs)
+    sizes = np.array([len(r) for r, _, _ in ens])
+    pick = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))} | set(W.rng(11).choice(256, 2, replace=False).tolist()))
+    print("case: full-size ensemble, oracle instances %s" % pick, flush=True)
+    sub = [ens[i] for i in pick]
+    owant, _ = per_instance(oracle, sub, [part[i] for i in pick], [tables[i] for i in pick], 2, funcs=models)
+    for i, w in zip(pick, owant):
+        assert_bitwise(got[i], w)

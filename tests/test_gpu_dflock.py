@@ -8,13 +8,18 @@ travelled.  Gradients: relative error of the weight gradient and norm-wise
 relative error of the initial-velocity and initial-position gradients
 <= 1e-4 when the FP64 adjoint runs along the GPU's own FP32 trajectory with the
 same upstream gradient: that pins the adjoint kernels.  Against the FP64
-model's own trajectory the weight gradient still agrees to 1e-4 and the
-per-agent gradients to 1e-3: after 50 steps the FP32 trajectory has drifted
-from the FP64 one by about 2e-4 of the distance travelled, and each side forms
-2 (p_T - target) / 3n from its own p_T, whose FP32 rounding is of the
-residual's size once the fit is close.  FP32 central differences (eps
-1e-3) agree with the adjoint to 1e-2 relative: the resolution of an FP32 loss
-difference."""
+model's own trajectory the weight gradient still agrees to 1e-4.  The
+per-agent gradients differ from the FP64 model's by what the FP32 trajectory
+itself implies: the FP64 adjoint run along the GPU's FP32 trajectory, with
+the GPU's upstream gradient, differs from the FP64 model's gradient by the
+same 2-3e-4 (50 steps), so the test asserts the GPU's end-to-end error
+within that drift bound + 1e-4 (and below 1e-3).  A shorter trajectory does
+not tighten this: the residual p_T - target shrinks with it faster than the
+trajectories part (5 steps: 6e-4).  Central differences of the FP32
+forward with the loss summed in float64: at eps 1e-3 every weight's
+difference quotient is within 2e-4 of the gradient's norm (the separation
+weight's own derivative is a hundred times smaller than the cohesion
+weight's), at eps 3e-3 within 1e-3 of its own value."""
 import numpy as np
 import pytest
 import torch
@@ -122,13 +127,23 @@ def test_weight_free_motion_closed_form(gpu):
 
 
 def test_gradient_matches_fp64_oracle_full_size(gpu, oracle64):
-    """65,536 agents in 16 clusters, 50 steps, checkpoints every 10."""
+    """65,536 agents in 16 clusters, 50 steps, checkpoints every 10: the
+    weight gradient against the FP64 model's to 1e-4; the per-agent
+    gradients within the bound the FP32 trajectory implies (the FP64
+    adjoint along the GPU's own trajectory, same upstream gradient, against
+    the FP64 model's gradient) + 1e-4."""
     n, steps = 65536, 50
     p0, v0 = problem(n, steps, seed=34)
     target = target_of(oracle64, p0, v0, steps)
     print("case: dflock gradient n=65536 steps=50 every=10 vs FP64", flush=True)
     with DiffFlock(gpu, n, R, H2, DT, DAMP, checkpoint_every=10) as f:
+        sp, sv = [p0], [v0]
+        for _ in range(steps):
+            ps, vs = f.forward(sp[-1], sv[-1], GUESS, 1)
+            sp.append(ps)
+            sv.append(vs)
         p, _ = f.forward(p0, v0, GUESS, steps)
+        assert np.array_equal(p, sp[-1])
         gp = (2.0 * (p.astype(np.float64) - target) / p.size).astype(np.float32)
         gp0, gv0, gw = f.backward(gp)
     loss64, gp64, gv64, gw64 = gradient64(oracle64, p0, v0, GUESS, PRM64, steps, target)
@@ -136,8 +151,11 @@ def test_gradient_matches_fp64_oracle_full_size(gpu, oracle64):
     werr = np.abs(gw - gw64) / np.abs(gw64)
     print("end to end: weight gradient", gw, gw64, "rel", werr, "v0 rel", rel(gv0, gv64), "p0 rel", rel(gp0, gp64))
     assert (werr <= 1e-4).all(), (gw, gw64)
-    assert rel(gv0, gv64) <= 1e-3
-    assert rel(gp0, gp64) <= 1e-3
+    ap, av, _ = backprop_along64(oracle64, np.stack(sp), np.stack(sv), GUESS, PRM64, gp)
+    bound_v, bound_p = rel(av, gv64), rel(ap, gp64)
+    print("drift bound (FP64 adjoint along the FP32 trajectory): v0 %.2e p0 %.2e" % (bound_v, bound_p))
+    assert rel(gv0, gv64) <= bound_v + 1e-4 and rel(gv0, gv64) <= 1e-3
+    assert rel(gp0, gp64) <= bound_p + 1e-4 and rel(gp0, gp64) <= 1e-3
     # the same upstream gradient, along the FP64 trajectory
     bp, bv, bw = backprop64(oracle64, p0, v0, GUESS, PRM64, steps, gp)
     print("same upstream: weight rel", np.abs(gw - bw) / np.abs(bw), "v0 rel", rel(gv0, bv), "p0 rel", rel(gp0, bp))
@@ -184,20 +202,25 @@ def test_checkpointing_does_not_change_the_gradient(gpu, oracle64):
             assert np.array_equal(a, b)
 
 
-def test_weight_gradient_matches_fp32_central_differences(gpu, oracle64):
+def test_weight_gradient_matches_central_differences(gpu, oracle64):
+    """Central differences of the GPU's FP32 forward, the loss summed in
+    float64 (mse), against the GPU adjoint."""
     n, steps = 16384, 20
     p0, v0 = problem(n, steps, seed=36)
     target = target_of(oracle64, p0, v0, steps)
-    eps = 1e-3
+    print("case: dflock central differences n=16384 steps=20", flush=True)
     with DiffFlock(gpu, n, R, H2, DT, DAMP) as f:
         p, _ = f.forward(p0, v0, GUESS, steps)
         _, _, gw = f.backward((2.0 * (p.astype(np.float64) - target) / p.size).astype(np.float32))
-        for k in range(3):
-            wp, wm = GUESS.copy(), GUESS.copy()
-            wp[k] += eps
-            wm[k] -= eps
-            fd = (mse(f.forward(p0, v0, wp, steps)[0], target) - mse(f.forward(p0, v0, wm, steps)[0], target)) / (2 * eps)
-            assert abs(gw[k] - fd) <= 1e-2 * abs(fd), (k, gw[k], fd)
+        for eps, bound in ((1e-3, lambda fd: 2e-4 * np.linalg.norm(gw)), (3e-3, lambda fd: 1e-3 * abs(fd))):
+            for k in range(3):
+                wp, wm = GUESS.copy(), GUESS.copy()
+                wp[k] += eps
+                wm[k] -= eps
+                fd = (mse(f.forward(p0, v0, wp, steps)[0], target) -
+                      mse(f.forward(p0, v0, wm, steps)[0], target)) / (2 * eps)
+                print("eps %g weight %d: adjoint %.8g difference quotient %.8g" % (eps, k, gw[k], fd))
+                assert abs(gw[k] - fd) <= bound(fd), (eps, k, gw[k], fd)
 
 
 def test_autograd_harness_matches_direct_call(gpu, oracle64):
