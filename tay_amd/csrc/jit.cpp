@@ -154,11 +154,13 @@ std::string compose_head(const char *const *sources, unsigned sources_count) {
 }
 
 /* Register budget of the see kernels, as wavefronts per SIMD the compiler
- * must fit.  The plain see kernel asks for 8 (at most 64 VGPRs): its compact
+ * must fit.  The plain see kernel asks for 7 (at most 72 VGPRs): its compact
  * loop is vector-issue bound and hides its LDS and L2 latency with more
- * wavefronts, and what does not fit is lane-group set-up state and the
- * sparse per-lane walk, spilled outside the compact loop (1M flocking, first
- * steps: 5.25 ms at 6, 5.00 ms at 8; SPH forces 1.20 -> 1.16 ms).  The
+ * wavefronts; what does not fit is seer state kept in scratch across the
+ * lane-group loop (flock_see: 80 bytes per lane at 7, 112 at 8).  1M
+ * flocking, see pass on the first steps: 5.04 ms at 7, 5.09 ms at 8, 5.25
+ * at 6; steps 196-200: 20.15 ms at 7, 20.44 at 8; per-launch HBM traffic
+ * 322 MB at 7 against 408 MB at 8; SPH forces 1.36 ms at 7, 1.34 at 8.  The
  * persistent batch kernel keeps the compiler's choice: its sparse instances
  * walk per lane, scattered loads that more resident wavefronts only make
  * miss more.  TAY_SEE_WAVES_PER_EU=<n> (1-8) overrides both, 0 leaves both
@@ -175,7 +177,7 @@ std::string see_occupancy(int fallback) {
 std::string compose(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
                     unsigned shapes_count, int count_pairs) {
     std::string text = compose_head(sources, sources_count);
-    const std::string occupancy = see_occupancy(8);
+    const std::string occupancy = see_occupancy(7);
     for (unsigned i = 0; i < shapes_count; ++i) {
         const TayGpuPassShape &p = shapes[i];
         if (p.type == TAY_GPU_PASS_SEE) {

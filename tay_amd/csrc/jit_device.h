@@ -579,17 +579,20 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
         tay_add_pairs(args.pair_count, pairs);
 }
 
-/* Batched see kernel: a persistent grid whose wavefronts take tiles from a
- * work queue until it is empty; every wavefront reaches the exit test after
+/* Batched see kernel: a persistent grid whose wavefronts take tiles in the
+ * given order (heaviest first).  Each wavefront's first tile is fixed: the
+ * w-th of the order for wavefront w, and the hardware places consecutive
+ * workgroups on different compute units, so the heaviest tiles start one
+ * per SIMD instead of wherever the first atomic requests happen to land;
+ * after that, tiles come from a work queue (starting after the first
+ * round) until it is empty.  Every wavefront reaches the exit test after
  * each tile, so the grid drains. */
 template <class T, unsigned WORDS, bool SKIP_SELF, int DIMS, bool COUNT, unsigned CTX_WORDS, class Call>
 __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call call) {
     unsigned pairs = 0;
+    const unsigned waves = gridDim.x * (TAY_SEE_BLOCK / 64u);
+    unsigned k = blockIdx.x * (TAY_SEE_BLOCK / 64u) + (threadIdx.x >> 6);
     for (;;) {
-        unsigned k = 0;
-        if ((threadIdx.x & 63u) == 0u)
-            k = atomicAdd(args.queue, 1u);
-        k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
         if (k >= args.tiles)
             break;
         const unsigned tile = args.order[k];
@@ -612,6 +615,10 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
             v.radii[d] = args.radii[4u * b + d];
         pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS>(
             v, tile * 64u, args.width, args.lane_walk, call);
+        unsigned q = 0;
+        if ((threadIdx.x & 63u) == 0u)
+            q = atomicAdd(args.queue, 1u);
+        k = waves + __builtin_amdgcn_readfirstlane(__shfl(q, 0));
     }
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);
