@@ -82,14 +82,25 @@ SMOOTH_F_PAIR = 36
 # the accumulators and counts (11 words); the seen side reads p and v (6).
 B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
 # HBM bytes per see launch on the headline workload, from two rocprofv3
-# --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by
-# what the same counter reads for tay_gpu_copy_probe's 1 GiB copy (FETCH x
-# 2.000, WRITE x 1.000; profiles/see_flock1M_pmc.md).  A counter run cannot
-# be made from inside this process: the figure is carried here, with the
-# commit of the kernel it was measured on, and re-measured whenever the see
-# kernel changes.
-SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of commit 9368d15",
-               "bytes": int((96527.0 * 2.000 + 216079.0 * 1.000) * 1024)}
+# --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch, means of the timed
+# state's dispatches), each scaled by what the same counter reads for
+# tay_gpu_copy_probe's 1 GiB copy (FETCH x 2.000, WRITE x 1.000;
+# profiles/see_flock1M_pmc.md).  A counter run cannot be made from inside
+# this process: the figure is carried here, with the commit of the kernel it
+# was measured on, and re-measured whenever the see kernel changes.
+SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of commit 5ff67bc",
+               "bytes": int((78003.9 * 2.000 + 158381.8 * 1.000) * 1024)}
+# The SPH line's see passes on 2M particles (profiles/see_sph2M_pmc.md),
+# same calibration.  Algorithmic bytes per particle: forces, the seer reads
+# p, v, pressure and both accumulators (15 words) and writes the
+# accumulators (8), the seen side reads p, v, density and pressure (8);
+# density, the seer reads p and density (4) and writes density (1), the seen
+# side reads p (3).
+SPH_TRAFFIC = {"particles": 2_000_000, "kernel": "see kernels of commit 5ff67bc",
+               "forces": int((83910.5 * 2.000 + 93749.2 * 1.000) * 1024),
+               "density": int((21017.3 * 2.000 + 7812.5 * 1.000) * 1024)}
+B_SPH_FORCES_PER_PARTICLE = 4 * (15 + 8 + 8)
+B_SPH_DENSITY_PER_PARTICLE = 4 * (4 + 1 + 3)
 
 
 def advance(sim, steps, name):
@@ -558,6 +569,7 @@ def bench_sph(lib, args):
         inside.append(((ball + len(a)) / (box + len(a)), ball / box))
     share_d = 0.5 * (inside[0][0] + inside[1][0])
     share_f = 0.5 * (inside[0][1] + inside[1][1])
+    measured = len(recs) == SPH_TRAFFIC["particles"]
     f_d = f_pair_of(SPH_DENSITY_F, {"inside": share_d})
     f_f = f_pair_of(SPH_FORCES_F, {"inside": share_f})
     out = {"workload": "SPH dam break, %d particles" % len(recs), "particles": len(recs), "h": 1.0, "dt": 1e-3,
@@ -567,10 +579,15 @@ def bench_sph(lib, args):
            "see_ms_per_step": [t / steps for t in see], "act_ms_per_step": [t / steps for t in act],
            "roofline": roofline(pairs[1] / steps * f_f, see[1] / 1e3 / steps, "see (tay_pass_2, sph_forces)", f_f,
                                 "the force pass, the longer of the two see kernels; share of box pairs inside h "
-                                "measured on the window's first and last particles",
+                                "measured on the window's first and last particles; traffic from rocprofv3 "
+                                "FETCH_SIZE/WRITE_SIZE (profiles/see_sph2M_pmc.md)",
+                                traffic=SPH_TRAFFIC["forces"] if measured else None,
+                                traffic_bytes_algorithmic=B_SPH_FORCES_PER_PARTICLE * len(recs),
                                 branch_shares={"inside": share_f},
                                 density=roofline(pairs[0] / steps * f_d, see[0] / 1e3 / steps,
                                                  "see (tay_pass_0, sph_density)", f_d, "",
+                                                 traffic=SPH_TRAFFIC["density"] if measured else None,
+                                                 traffic_bytes_algorithmic=B_SPH_DENSITY_PER_PARTICLE * len(recs),
                                                  branch_shares={"inside": share_d}))}
     if not args.no_cpu_baseline:
         base = {}

@@ -121,10 +121,14 @@ __device__ __forceinline__ unsigned group_exclusive_count(unsigned v, unsigned s
  * neighbours (their key-order slots) to its own list of KL entries with its
  * count, everything indexed by the agent's key-order slot t; entry e of
  * lane l is word (t * KL + e) * LANES + l, so the lanes of an agent that
- * read their e-th entries together read one contiguous run.  max_count[0] takes the largest
- * neighbour count of an agent, max_count[1] the largest of one lane, which
- * sizes the lists of the recording recomputation; pair_total (when given)
- * adds up every agent's neighbour count. */
+ * read their e-th entries together read one contiguous run.  Block b keeps
+ * running statistics in stats[4b..4b+3] across the steps of a run: the
+ * largest neighbour count of an agent, the largest of one lane (which sizes
+ * the lists of the recording recomputation) and the 64-bit sum of every
+ * agent's neighbour count.  Each block updates only its own slot, after the
+ * previous step's launch has finished with it, so no atomics are needed
+ * (one slot per wavefront on a single address serialised the whole grid);
+ * k_df_stats_reduce folds the slots when a caller asks. */
 template <bool RECORD>
 __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols, unsigned n,
                                                  const unsigned *__restrict__ ids, const float *__restrict__ srt,
@@ -132,8 +136,7 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
                                                  const TayGridGeom *__restrict__ g, Prm c,
                                                  float *__restrict__ out, unsigned *__restrict__ nbr, unsigned KL,
                                                  unsigned *__restrict__ counts, float *__restrict__ sums,
-                                                 unsigned *__restrict__ max_count,
-                                                 unsigned long long *__restrict__ pair_total) {
+                                                 unsigned *__restrict__ stats) {
     const unsigned t = (blockIdx.x * 256u + threadIdx.x) / LANES;
     const unsigned sub = threadIdx.x % LANES;
     const bool live = t < n;
@@ -210,17 +213,64 @@ __global__ __launch_bounds__(256) void k_df_step(const float *__restrict__ cols,
     }
     const unsigned m = wave_max_u(live ? total : 0u);
     const unsigned ml = wave_max_u(live ? cnt : 0u);
-    if ((threadIdx.x & 63u) == 0u && m) {
-        atomicMax(max_count, m);
-        atomicMax(max_count + 1, ml);
+    /* each agent's count once (its lanes hold partial counts) */
+    unsigned own = live ? cnt : 0u;
+    for (int o = 32; o > 0; o >>= 1)
+        own += __shfl_xor(own, o);
+    __shared__ unsigned wave_stats[256 / 64][3];
+    const unsigned w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0u) {
+        wave_stats[w][0] = m;
+        wave_stats[w][1] = ml;
+        wave_stats[w][2] = own;
     }
-    if (pair_total) {
-        /* each agent's count once (its lanes hold partial counts) */
-        unsigned long long own = live ? cnt : 0u;
-        for (int o = 32; o > 0; o >>= 1)
-            own += __shfl_xor(own, o);
-        if ((threadIdx.x & 63u) == 0u && own)
-            atomicAdd(pair_total, own);
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        unsigned bm = 0, bml = 0;
+        unsigned long long bsum = 0;
+        for (unsigned k = 0; k < 256 / 64; ++k) {
+            bm = wave_stats[k][0] > bm ? wave_stats[k][0] : bm;
+            bml = wave_stats[k][1] > bml ? wave_stats[k][1] : bml;
+            bsum += wave_stats[k][2];
+        }
+        unsigned *slot = stats + 4ull * blockIdx.x;
+        slot[0] = slot[0] > bm ? slot[0] : bm;
+        slot[1] = slot[1] > bml ? slot[1] : bml;
+        *(unsigned long long *)(slot + 2) += bsum;
+    }
+}
+
+/* Folds the per-block statistics of k_df_step: out[0] = largest neighbour
+ * count of an agent, out[1] = largest of one lane, out[2..3] = the 64-bit
+ * pair total.  One block of 256 threads. */
+__global__ __launch_bounds__(256) void k_df_stats_reduce(const unsigned *__restrict__ stats, unsigned blocks,
+                                                         unsigned *__restrict__ out) {
+    unsigned m = 0, ml = 0;
+    unsigned long long sum = 0;
+    for (unsigned b = threadIdx.x; b < blocks; b += 256u) {
+        const unsigned *slot = stats + 4ull * b;
+        m = slot[0] > m ? slot[0] : m;
+        ml = slot[1] > ml ? slot[1] : ml;
+        sum += *(const unsigned long long *)(slot + 2);
+    }
+    __shared__ unsigned sm[256], sml[256];
+    __shared__ unsigned long long ssum[256];
+    sm[threadIdx.x] = m;
+    sml[threadIdx.x] = ml;
+    ssum[threadIdx.x] = sum;
+    __syncthreads();
+    for (unsigned h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            sm[threadIdx.x] = sm[threadIdx.x + h] > sm[threadIdx.x] ? sm[threadIdx.x + h] : sm[threadIdx.x];
+            sml[threadIdx.x] = sml[threadIdx.x + h] > sml[threadIdx.x] ? sml[threadIdx.x + h] : sml[threadIdx.x];
+            ssum[threadIdx.x] += ssum[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0u) {
+        out[0] = sm[0];
+        out[1] = sml[0];
+        *(unsigned long long *)(out + 2) = ssum[0];
     }
 }
 
@@ -442,7 +492,9 @@ struct TayDflock {
     TayGridGeom *geom;
     unsigned *keys[2], *vals[2], *block_hist, *scan_sums;
     TayCellRange *ranges;
-    unsigned *max_count;
+    unsigned *max_count;        /* folded statistics: two maxima and the 64-bit pair total */
+    unsigned *stats[2];         /* per-block statistics of the forward [0] and of the recording [1] */
+    unsigned stats_blocks;
     /* recorded segment */
     float *seg_states;          /* every x 8 words per agent */
     unsigned *seg_ids;          /* every x n: agents in key order */
@@ -531,7 +583,8 @@ TAY_API void tay_dflock_destroy(TayDflock *f) {
     for (float *c : f->ckpt)
         device_free(c);
     void *bufs[] = {f->cols[0], f->cols[1], f->box_partials, f->geom, f->keys[0], f->keys[1], f->vals[0],
-                    f->vals[1], f->block_hist, f->scan_sums, f->ranges, f->max_count, f->seg_states, f->seg_ids,
+                    f->vals[1], f->block_hist, f->scan_sums, f->ranges, f->max_count, f->stats[0], f->stats[1],
+                    f->seg_states, f->seg_ids,
                     f->seg_sums, f->seg_counts, f->seg_nbr, f->Lp[0], f->Lp[1], f->Lv[0], f->Lv[1], f->adj,
                     f->gpart, f->gw, f->gparts, f->io, f->srt, f->seg_srt};
     for (void *p : bufs)
@@ -565,6 +618,7 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
     f->n = n;
     f->params = *params;
     f->every = checkpoint_every;
+    f->stats_blocks = (unsigned)(((unsigned long long)n * LANES + 255u) / 256u);
     f->max_cells = max_cells;
     f->box_blocks = blocks_for(n, kBuildBlock * 4, 1024);
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
@@ -582,6 +636,8 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
               device_alloc((void **)&f->scan_sums, 4ull * kScanBlocks, "scan sums") &&
               device_alloc((void **)&f->ranges, sizeof(TayCellRange) * max_cells, "cell table") &&
               device_alloc((void **)&f->max_count, 16, "neighbour maxima and pair total") &&
+              device_alloc((void **)&f->stats[0], 16ull * f->stats_blocks, "forward statistics") &&
+              device_alloc((void **)&f->stats[1], 16ull * f->stats_blocks, "recording statistics") &&
               device_alloc((void **)&f->seg_states, state_bytes(f) * checkpoint_every, "segment states") &&
               device_alloc((void **)&f->seg_ids, 4ull * n * checkpoint_every, "segment orders") &&
               device_alloc((void **)&f->seg_sums, 10ull * n * 4 * checkpoint_every, "segment sums") &&
@@ -601,6 +657,15 @@ TAY_API TayDflock *tay_dflock_create(unsigned n, const TayDflockParams *params, 
         return nullptr;
     }
     return f;
+}
+
+/* Folds the forward's (which = 0) or the recording's (1) per-block
+ * statistics into max_count and copies them to out[4] (synchronises). */
+static bool read_stats(TayDflock *f, int which, unsigned *out, hipStream_t s) {
+    k_df_stats_reduce<<<1, 256, 0, s>>>(f->stats[which], f->stats_blocks, f->max_count);
+    return launch_ok("statistics") &&
+           hip_ok(hipMemcpyAsync(out, f->max_count, 16, hipMemcpyDeviceToHost, s), "read neighbour statistics") &&
+           hip_ok(hipStreamSynchronize(s), "read neighbour statistics");
 }
 
 TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, const float *weights,
@@ -626,7 +691,7 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
     ok = ok && hip_ok(hipMemcpyAsync(f->io, v0, rows3, hipMemcpyHostToDevice, s), "upload v0");
     if (ok)
         k_df_unpack<<<blocks, 256, 0, s>>>(f->io, n, f->cols[0] + 5ull * n);
-    ok = ok && hip_ok(hipMemsetAsync(f->max_count, 0, 16, s), "hipMemsetAsync") &&
+    ok = ok && hip_ok(hipMemsetAsync(f->stats[0], 0, 16ull * f->stats_blocks, s), "hipMemsetAsync") &&
          hip_ok(hipStreamSynchronize(s), "upload initial state");
     if (!ok)
         return 0;
@@ -642,8 +707,7 @@ TAY_API int tay_dflock_forward(TayDflock *f, const float *p0, const float *v0, c
         df_sort(f, f->cols[cur], s);
         k_df_sorted_copy<<<blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->srt);
         k_df_step<false><<<lane_blocks, 256, 0, s>>>(f->cols[cur], n, f->vals[0], f->srt, f->ranges, f->geom, c,
-                                                f->cols[cur ^ 1], nullptr, 0, nullptr, nullptr, f->max_count,
-                                                (unsigned long long *)(f->max_count + 2));
+                                                f->cols[cur ^ 1], nullptr, 0, nullptr, nullptr, f->stats[0]);
         cur ^= 1;
     }
     ok = ok && launch_ok("differentiable forward") && timed_end(ev, s, &f->forward_ms);
@@ -674,9 +738,8 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
     const unsigned n = f->n, blocks = (n + 255u) / 256u;
     const unsigned lane_blocks = (unsigned)(((unsigned long long)n * LANES + 255u) / 256u);
     const size_t rows3 = 3ull * n * sizeof(float);
-    unsigned maxc[2] = {0u, 0u};
-    bool ok = hip_ok(hipMemcpyAsync(maxc, f->max_count, 8, hipMemcpyDeviceToHost, s), "read neighbour maxima") &&
-              hip_ok(hipStreamSynchronize(s), "read neighbour maxima");
+    unsigned maxc[4] = {0u, 0u, 0u, 0u};
+    bool ok = read_stats(f, 0, maxc, s);
     if (!ok)
         return 0;
     const unsigned KL = maxc[1] ? maxc[1] : 1u;
@@ -701,7 +764,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
         ok = ok && hip_ok(hipMemsetAsync(f->Lv[0], 0, rows3, s), "hipMemsetAsync");
     }
     ok = ok && hip_ok(hipMemsetAsync(f->gw, 0, 24, s), "hipMemsetAsync") &&
-         hip_ok(hipMemsetAsync(f->max_count, 0, 8, s), "hipMemsetAsync") &&
+         hip_ok(hipMemsetAsync(f->stats[1], 0, 16ull * f->stats_blocks, s), "hipMemsetAsync") &&
          hip_ok(hipStreamSynchronize(s), "upload gradient");
     if (!ok)
         return 0;
@@ -729,7 +792,7 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
             k_df_step<true><<<lane_blocks, 256, 0, s>>>(x, n, f->vals[0], srt, f->ranges, f->geom, c, next,
                                                    f->seg_nbr + (size_t)r * KL * LANES * n, KL,
                                                    f->seg_counts + (size_t)r * LANES * n,
-                                                   f->seg_sums + (size_t)r * 10 * n, f->max_count, nullptr);
+                                                   f->seg_sums + (size_t)r * 10 * n, f->stats[1]);
         }
         /* reverse the segment's steps */
         for (unsigned t = t1; ok && t-- > t0;) {
@@ -748,9 +811,8 @@ TAY_API int tay_dflock_backward(TayDflock *f, const float *grad_p_final, const f
         ok = ok && launch_ok("differentiable backward");
     }
     ok = ok && timed_end(ev, s, &f->backward_ms);
-    unsigned recount[2] = {0u, 0u};
-    ok = ok && hip_ok(hipMemcpyAsync(recount, f->max_count, 8, hipMemcpyDeviceToHost, s), "read neighbour maxima") &&
-         hip_ok(hipStreamSynchronize(s), "read neighbour maxima");
+    unsigned recount[4] = {0u, 0u, 0u, 0u};
+    ok = ok && read_stats(f, 1, recount, s);
     if (ok && recount[1] > KL) {
         set_error("recomputed per-lane neighbour count %u exceeds the forward's %u", recount[1], KL);
         return 0;
@@ -782,12 +844,18 @@ TAY_API int tay_dflock_times(TayDflock *f, double *forward_ms, double *backward_
 
 TAY_API int tay_dflock_max_neighbours(TayDflock *f, unsigned *count) {
     hipStream_t s = stream();
-    return s && hip_ok(hipMemcpyAsync(count, f->max_count, 4, hipMemcpyDeviceToHost, s), "read neighbour maximum") &&
-           hip_ok(hipStreamSynchronize(s), "read neighbour maximum");
+    unsigned st[4];
+    if (!s || !read_stats(f, 0, st, s))
+        return 0;
+    *count = st[0];
+    return 1;
 }
 
 TAY_API int tay_dflock_pair_count(TayDflock *f, unsigned long long *pairs) {
     hipStream_t s = stream();
-    return s && hip_ok(hipMemcpyAsync(pairs, f->max_count + 2, 8, hipMemcpyDeviceToHost, s), "read pair total") &&
-           hip_ok(hipStreamSynchronize(s), "read pair total");
+    unsigned st[4];
+    if (!s || !read_stats(f, 0, st, s))
+        return 0;
+    std::memcpy(pairs, st + 2, sizeof(*pairs));
+    return 1;
 }

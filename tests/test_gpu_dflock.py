@@ -272,3 +272,30 @@ def test_calibrates_weights_to_a_target_trajectory(gpu, oracle64):
     print("calibration: loss %.4g -> %.4g, weights %s (hidden %s)" % (loss0, loss1, w, HIDDEN))
     assert loss1 < 1e-2 * loss0
     assert np.allclose(w, HIDDEN, rtol=0.05)
+
+
+def brute_counts(p):
+    """Per-agent neighbour counts: float32 offsets and squared lengths in
+    the kernel's order, r^2 < R^2 strictly, the agent itself excluded."""
+    p = np.asarray(p, np.float32)
+    d = p[None, :, :] - p[:, None, :]
+    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    inside = r2 < np.float32(R) * np.float32(R)
+    np.fill_diagonal(inside, False)
+    return inside.sum(axis=1)
+
+
+def test_neighbour_statistics_match_brute_force(gpu):
+    """The pair total and the largest neighbour count of a forward run equal
+    brute-force counts over the positions each step starts from."""
+    p0, v0 = problem(3000, 2, seed=35, clusters=6, box=250.0)
+    print("case: dflock neighbour statistics n=3000 steps=1,2", flush=True)
+    with DiffFlock(gpu, len(p0), R, H2, DT, DAMP) as f:
+        p1, _ = f.forward(p0, v0, GUESS, 1)
+        c0 = brute_counts(p0)
+        assert f.pair_count() == int(c0.sum()) > 0
+        assert f.max_neighbours() == int(c0.max())
+        f.forward(p0, v0, GUESS, 2)
+        c1 = brute_counts(p1)
+        assert f.pair_count() == int(c0.sum() + c1.sum())
+        assert f.max_neighbours() == int(max(c0.max(), c1.max()))
