@@ -48,7 +48,8 @@ def assert_bitwise(a, b, fields=None):
 
 def xyz_fields(dtype):
     """Fields to compare bitwise; the unused .w lane of float4 fields is
-    compared too since every model helper writes it as 0."""
+    compared too: the model helpers define it (a result keeps its first
+    operand's w), so the GPU and the oracle must agree on it as well."""
     return list(dtype.names)
 
 
