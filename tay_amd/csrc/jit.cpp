@@ -154,34 +154,43 @@ std::string compose_head(const char *const *sources, unsigned sources_count) {
 }
 
 /* Register budget of the see kernels, as wavefronts per SIMD the compiler
- * must fit.  The plain see kernel asks for 7 (at most 72 VGPRs): its compact
- * loop is vector-issue bound and hides its LDS and L2 latency with more
- * wavefronts; what does not fit is seer state kept in scratch across the
- * lane-group loop (flock_see: 80 bytes per lane at 7, 112 at 8).  1M
- * flocking, see pass on the first steps: 5.04 ms at 7, 5.09 ms at 8, 5.25
- * at 6; steps 196-200: 20.15 ms at 7, 20.44 at 8; per-launch HBM traffic
- * 322 MB at 7 against 408 MB at 8; SPH forces 1.36 ms at 7, 1.34 at 8.  The
- * persistent batch kernel keeps the compiler's choice: its sparse instances
- * walk per lane, scattered loads that more resident wavefronts only make
- * miss more.  TAY_SEE_WAVES_PER_EU=<n> (1-8) overrides both, 0 leaves both
- * to the compiler. */
-std::string see_occupancy(int fallback) {
-    int n = fallback;
+ * must fit.  A plain see kernel is compiled for 8 (at most 64 VGPRs) and, if
+ * it then keeps values in scratch, for 7 (72 VGPRs): its compact loop is
+ * vector-issue bound and hides its LDS and L2 latency with more wavefronts,
+ * but scratch costs HBM traffic and the moves in and out of it.  Measured:
+ * sph_forces / sph_density have no scratch at 8 and run 2-5 % faster there
+ * than at 7; flock_see keeps 40 bytes per lane at 8 and none at 7, where the
+ * 1M-agent see pass moves 125 MB instead of 207 MB at equal time (first
+ * steps 4.92 ms at 7, 4.97 at 8; 5.26 at 6).  The persistent batch kernel
+ * keeps the compiler's choice: its sparse instances walk per lane, scattered
+ * loads that more resident wavefronts only make miss more.
+ * TAY_SEE_WAVES_PER_EU=<n> (1-8) fixes the budget of every see kernel, 0
+ * leaves it to the compiler. */
+constexpr int kSeeWavesFirst = 8;
+constexpr int kSeeWavesSpilled = 7;
+
+/* The budget from TAY_SEE_WAVES_PER_EU, or -1 when it is not set. */
+int see_waves_override() {
     if (const char *w = std::getenv("TAY_SEE_WAVES_PER_EU"))
-        n = std::atoi(w);
+        return std::atoi(w);
+    return -1;
+}
+
+std::string see_occupancy(int n) {
     if (n >= 1 && n <= 8)
         return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
     return std::string();
 }
 
+/* waves[i]: register budget of see pass i (see_occupancy) */
 std::string compose(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
-                    unsigned shapes_count, int count_pairs) {
+                    unsigned shapes_count, int count_pairs, const int *waves) {
     std::string text = compose_head(sources, sources_count);
-    const std::string occupancy = see_occupancy(7);
     for (unsigned i = 0; i < shapes_count; ++i) {
         const TayGpuPassShape &p = shapes[i];
         if (p.type == TAY_GPU_PASS_SEE) {
-            append(text, "extern \"C\" __global__ __launch_bounds__(%u)%s", kSeeBlock, occupancy.c_str());
+            append(text, "extern \"C\" __global__ __launch_bounds__(%u)%s", kSeeBlock,
+                   see_occupancy(waves[i]).c_str());
             append(text,
                    " void tay_pass_%u(TaySeeArgs args) {\n"
                    "    typedef tay_see_traits<decltype(%s)> T;\n"
@@ -265,7 +274,8 @@ bool tay::compile_program(const std::string &text, std::vector<char> &code, char
 std::string tay::compose_batch(const char *const *sources, unsigned sources_count, const TayGpuBatchPass *passes,
                                unsigned passes_count, unsigned agent_size, int dims, int count_pairs) {
     std::string text = compose_head(sources, sources_count);
-    const std::string occupancy = see_occupancy(0);
+    const int forced = see_waves_override();
+    const std::string occupancy = see_occupancy(forced >= 0 ? forced : 0);
     for (unsigned i = 0; i < passes_count; ++i) {
         const TayGpuBatchPass &p = passes[i];
         if (p.type == TAY_GPU_PASS_SEE) {
@@ -309,7 +319,9 @@ TAY_API int tay_gpu_compile_check(const char *const *sources, unsigned sources_c
         }
     }
     std::vector<char> code;
-    if (!compile_program(compose(sources, sources_count, shapes, shapes_count, 0), code, log, log_size))
+    const int forced = see_waves_override();
+    std::vector<int> waves(shapes_count, forced >= 0 ? forced : kSeeWavesFirst);
+    if (!compile_program(compose(sources, sources_count, shapes, shapes_count, 0, waves.data()), code, log, log_size))
         return 0;
     if (code_size)
         *code_size = code.size();
@@ -364,8 +376,11 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
     std::vector<TayGpuPassShape> shapes(passes_count);
     for (unsigned i = 0; i < passes_count; ++i)
         shapes[i] = shape_of(passes[i]);
+    const int forced = see_waves_override();
+    std::vector<int> waves(passes_count, forced >= 0 ? forced : kSeeWavesFirst);
     std::vector<char> code;
-    if (!compile_program(compose(sources, sources_count, shapes.data(), passes_count, count_pairs), code, log, log_size))
+    if (!compile_program(compose(sources, sources_count, shapes.data(), passes_count, count_pairs, waves.data()), code,
+                         log, log_size))
         return nullptr;
 
     TayGpuProgram *p = new TayGpuProgram();
@@ -377,6 +392,39 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
     if (!hip_ok(hipModuleLoadData(&p->module, code.data()), "hipModuleLoadData")) {
         tay_gpu_program_destroy(p);
         return nullptr;
+    }
+    if (forced < 0) {
+        /* see kernels that keep values in scratch at the first budget are
+         * compiled again for the second */
+        bool again = false;
+        for (unsigned i = 0; i < passes_count; ++i) {
+            if (passes[i].type != TAY_GPU_PASS_SEE)
+                continue;
+            char name[64];
+            std::snprintf(name, sizeof(name), "tay_pass_%u", i);
+            hipFunction_t fn = nullptr;
+            int scratch = 0;
+            if (!hip_ok(hipModuleGetFunction(&fn, p->module, name), "hipModuleGetFunction") ||
+                !hip_ok(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn), "hipFuncGetAttribute")) {
+                tay_gpu_program_destroy(p);
+                return nullptr;
+            }
+            if (scratch > 0) {
+                waves[i] = kSeeWavesSpilled;
+                again = true;
+            }
+        }
+        if (again) {
+            (void)hipModuleUnload(p->module);
+            p->module = nullptr;
+            if (!compile_program(compose(sources, sources_count, shapes.data(), passes_count, count_pairs,
+                                         waves.data()),
+                                 code, log, log_size) ||
+                !hip_ok(hipModuleLoadData(&p->module, code.data()), "hipModuleLoadData")) {
+                tay_gpu_program_destroy(p);
+                return nullptr;
+            }
+        }
     }
     p->passes.resize(passes_count);
     for (unsigned i = 0; i < passes_count; ++i) {

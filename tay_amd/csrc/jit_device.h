@@ -406,9 +406,13 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
 #pragma unroll
         for (int k = 0; k < DIMS; ++k)
             limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / g->cell_size[k]) + 3u;
+        /* wave-uniform values held in scalar registers, so that the group
+         * width and start and the split decision stay uniform too */
+        limit = __builtin_amdgcn_readfirstlane(limit);
         const unsigned cx = (unsigned)g->count[0];
         const unsigned cxy = cx * (unsigned)g->count[1];
-        const unsigned n_active = v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u;
+        const unsigned n_active =
+            __builtin_amdgcn_readfirstlane(v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u);
         unsigned beg = 0, width = width0;
         while (beg < n_active) {
             member = active && lane - beg < width;
@@ -445,11 +449,16 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                     int plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
 #pragma unroll
                     for (int k = 0; k < DIMS; ++k) {
-                        const float mlo = (fabsf(a_lo[k]) + r[k] + ext[k]) * 0x1p-20f;
-                        const float mhi = (fabsf(a_hi[k]) + r[k]) * 0x1p-20f;
-                        plo[k] = tay_cell_axis(((a_lo[k] - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k],
+                        /* computed here, not ahead of the lane-group loop
+                         * (where the compiler would hoist the seer-only
+                         * bounds and keep them in scratch for the whole tile) */
+                        float al = a_lo[k], ah = a_hi[k];
+                        asm volatile("" : "+v"(al), "+v"(ah));
+                        const float mlo = (fabsf(al) + r[k] + ext[k]) * 0x1p-20f;
+                        const float mhi = (fabsf(ah) + r[k]) * 0x1p-20f;
+                        plo[k] = tay_cell_axis(((al - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k],
                                                g->count[k]);
-                        phi[k] = tay_cell_axis((a_hi[k] + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
+                        phi[k] = tay_cell_axis((ah + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                     }
                     for (int x = plo[0]; x <= phi[0]; ++x)
                         for (int y = plo[1]; y <= phi[1]; ++y)
