@@ -86,6 +86,19 @@ __device__ __forceinline__ float tay_wave_max(float v) {
     return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
 }
 
+/* A wave-uniform value moved to a scalar register: loads from a uniform
+ * address after a store of the same kernel are vector loads, and values
+ * derived from them would otherwise stay in vector registers. */
+__device__ __forceinline__ float tay_uniform(float v) {
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+__device__ __forceinline__ int tay_uniform(int v) {
+    return (int)__builtin_amdgcn_readfirstlane((unsigned)v);
+}
+__device__ __forceinline__ unsigned tay_uniform(unsigned v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 __device__ __forceinline__ unsigned tay_wave_sum(unsigned v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
@@ -396,21 +409,25 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         /* Box agents are binned by their min corner: a seen agent whose box
          * reaches down to p - r has its min corner at most the group's
          * largest extent below that. */
-        float ext[3] = {0.0f, 0.0f, 0.0f};
-        if (SEEN_BOX) {
+        float ext[3] = {0.0f, 0.0f, 0.0f}, origin[3] = {0.0f, 0.0f, 0.0f}, cell[3] = {1.0f, 1.0f, 1.0f};
+        int count[3] = {1, 1, 1};
 #pragma unroll
-            for (int k = 0; k < DIMS; ++k)
-                ext[k] = g->max_extent[k];
+        for (int k = 0; k < DIMS; ++k) {
+            origin[k] = tay_uniform(g->origin[k]);
+            cell[k] = tay_uniform(g->cell_size[k]);
+            count[k] = tay_uniform(g->count[k]);
+            if (SEEN_BOX)
+                ext[k] = tay_uniform(g->max_extent[k]);
         }
         unsigned limit = 1;
 #pragma unroll
         for (int k = 0; k < DIMS; ++k)
-            limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / g->cell_size[k]) + 3u;
+            limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / cell[k]) + 3u;
         /* wave-uniform values held in scalar registers, so that the group
          * width and start and the split decision stay uniform too */
         limit = __builtin_amdgcn_readfirstlane(limit);
-        const unsigned cx = (unsigned)g->count[0];
-        const unsigned cxy = cx * (unsigned)g->count[1];
+        const unsigned cx = (unsigned)count[0];
+        const unsigned cxy = cx * (unsigned)count[1];
         const unsigned n_active =
             __builtin_amdgcn_readfirstlane(v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u);
         unsigned beg = 0, width = width0;
@@ -427,8 +444,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                  * cell holding a passing pair is missed */
                 float mlo = (fabsf(tmin) + r[k] + ext[k]) * 0x1p-20f;
                 float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
-                lo[k] = tay_cell_axis(((tmin - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
-                hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
+                lo[k] = tay_cell_axis(((tmin - r[k]) - ext[k]) - mlo, origin[k], cell[k], count[k]);
+                hi[k] = tay_cell_axis((tmax + r[k]) + mhi, origin[k], cell[k], count[k]);
                 cells *= (unsigned)(hi[k] - lo[k] + 1);
             }
             if (cells > limit && width > 1u && width > lane_walk) {
@@ -456,9 +473,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                         asm volatile("" : "+v"(al), "+v"(ah));
                         const float mlo = (fabsf(al) + r[k] + ext[k]) * 0x1p-20f;
                         const float mhi = (fabsf(ah) + r[k]) * 0x1p-20f;
-                        plo[k] = tay_cell_axis(((al - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k],
-                                               g->count[k]);
-                        phi[k] = tay_cell_axis((ah + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
+                        plo[k] = tay_cell_axis(((al - r[k]) - ext[k]) - mlo, origin[k], cell[k], count[k]);
+                        phi[k] = tay_cell_axis((ah + r[k]) + mhi, origin[k], cell[k], count[k]);
                     }
                     for (int x = plo[0]; x <= phi[0]; ++x)
                         for (int y = plo[1]; y <= phi[1]; ++y)
@@ -604,9 +620,13 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
     for (;;) {
         if (k >= args.tiles)
             break;
-        const unsigned tile = args.order[k];
-        const unsigned b = args.tile_inst[tile];
-        const TayBatchInstance inst = args.inst[b];
+        const unsigned tile = tay_uniform(args.order[k]);
+        const unsigned b = tay_uniform(args.tile_inst[tile]);
+        TayBatchInstance inst = args.inst[b];
+        inst.off = tay_uniform(inst.off);
+        inst.count = tay_uniform(inst.count);
+        inst.ranges_off = ((unsigned long long)tay_uniform((unsigned)(inst.ranges_off >> 32)) << 32) |
+                          tay_uniform((unsigned)inst.ranges_off);
         TaySeeView v;
         v.seer_cols = args.cols;
         v.seer_out = args.cols;
@@ -621,7 +641,7 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
         v.context = args.context + (unsigned long long)b * args.ctx_words;
 #pragma unroll
         for (int d = 0; d < 4; ++d)
-            v.radii[d] = args.radii[4u * b + d];
+            v.radii[d] = tay_uniform(args.radii[4u * b + d]);
         pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS>(
             v, tile * 64u, args.width, args.lane_walk, call);
         unsigned q = 0;
