@@ -409,25 +409,21 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         /* Box agents are binned by their min corner: a seen agent whose box
          * reaches down to p - r has its min corner at most the group's
          * largest extent below that. */
-        float ext[3] = {0.0f, 0.0f, 0.0f}, origin[3] = {0.0f, 0.0f, 0.0f}, cell[3] = {1.0f, 1.0f, 1.0f};
-        int count[3] = {1, 1, 1};
+        float ext[3] = {0.0f, 0.0f, 0.0f};
+        if (SEEN_BOX) {
 #pragma unroll
-        for (int k = 0; k < DIMS; ++k) {
-            origin[k] = tay_uniform(g->origin[k]);
-            cell[k] = tay_uniform(g->cell_size[k]);
-            count[k] = tay_uniform(g->count[k]);
-            if (SEEN_BOX)
-                ext[k] = tay_uniform(g->max_extent[k]);
+            for (int k = 0; k < DIMS; ++k)
+                ext[k] = g->max_extent[k];
         }
         unsigned limit = 1;
 #pragma unroll
         for (int k = 0; k < DIMS; ++k)
-            limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / cell[k]) + 3u;
+            limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / g->cell_size[k]) + 3u;
         /* wave-uniform values held in scalar registers, so that the group
          * width and start and the split decision stay uniform too */
         limit = __builtin_amdgcn_readfirstlane(limit);
-        const unsigned cx = (unsigned)count[0];
-        const unsigned cxy = cx * (unsigned)count[1];
+        const unsigned cx = (unsigned)g->count[0];
+        const unsigned cxy = cx * (unsigned)g->count[1];
         const unsigned n_active =
             __builtin_amdgcn_readfirstlane(v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u);
         unsigned beg = 0, width = width0;
@@ -444,8 +440,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                  * cell holding a passing pair is missed */
                 float mlo = (fabsf(tmin) + r[k] + ext[k]) * 0x1p-20f;
                 float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
-                lo[k] = tay_cell_axis(((tmin - r[k]) - ext[k]) - mlo, origin[k], cell[k], count[k]);
-                hi[k] = tay_cell_axis((tmax + r[k]) + mhi, origin[k], cell[k], count[k]);
+                lo[k] = tay_cell_axis(((tmin - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k], g->count[k]);
+                hi[k] = tay_cell_axis((tmax + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                 cells *= (unsigned)(hi[k] - lo[k] + 1);
             }
             if (cells > limit && width > 1u && width > lane_walk) {
@@ -473,8 +469,9 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                         asm volatile("" : "+v"(al), "+v"(ah));
                         const float mlo = (fabsf(al) + r[k] + ext[k]) * 0x1p-20f;
                         const float mhi = (fabsf(ah) + r[k]) * 0x1p-20f;
-                        plo[k] = tay_cell_axis(((al - r[k]) - ext[k]) - mlo, origin[k], cell[k], count[k]);
-                        phi[k] = tay_cell_axis((ah + r[k]) + mhi, origin[k], cell[k], count[k]);
+                        plo[k] = tay_cell_axis(((al - r[k]) - ext[k]) - mlo, g->origin[k], g->cell_size[k],
+                                               g->count[k]);
+                        phi[k] = tay_cell_axis((ah + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                     }
                     for (int x = plo[0]; x <= phi[0]; ++x)
                         for (int y = plo[1]; y <= phi[1]; ++y)
