@@ -212,14 +212,18 @@ struct TaySeeView {
  * Bits are taken in ascending order and chunks follow the candidate order,
  * so every seer still meets its pairs in the reference's sweep order.
  *
- * i0 is the tile's first slot; `width` (a power of two up to 64) is the
- * size of the lane groups a tile starts with; a group of at most lane_walk
+ * i0 is the tile's first slot and `seers` (64, 32 or 16) the number of
+ * seers it holds on lanes 0 to seers - 1 (fewer than 64 for small states,
+ * whose 64-seer tiles would leave most SIMDs idle; the other lanes still
+ * stage and filter candidates but own no seer); `width` (a power of two up
+ * to `seers`) is the size of the lane groups a tile starts with; a group of at most lane_walk
  * lanes whose box is still too wide walks per lane instead of splitting
  * further (0: always split).  Returns the pairs this lane passed to the
  * model when COUNT is set. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
           bool COUNT, unsigned CTX_WORDS, class Call>
-__device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned width0, unsigned lane_walk,
+__device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned seers, unsigned width0,
+                                                unsigned lane_walk,
                                                 Call call) {
     static_assert(sizeof(typename T::seer) <= SEER_WORDS * 4u, "seer record larger than the group's agent size");
     static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
@@ -229,11 +233,12 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
 
     __shared__ __align__(16) float chunk_lo[WAVES][64][4];
     __shared__ float chunk_hi[SEEN_BOX ? WAVES : 1u][64][4];
+    __shared__ __align__(8) unsigned cell_ranges[SEEN_GRID ? WAVES : 1u][64][2];
 
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
     const unsigned i = i0 + lane;
-    const bool active = i < v.seer_end;
+    const bool active = lane < seers && i < v.seer_end;
     const unsigned ia = active ? i : i0;
 
     union {
@@ -425,7 +430,7 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         const unsigned cx = (unsigned)g->count[0];
         const unsigned cxy = cx * (unsigned)g->count[1];
         const unsigned n_active =
-            __builtin_amdgcn_readfirstlane(v.seer_end - i0 < 64u ? v.seer_end - i0 : 64u);
+            __builtin_amdgcn_readfirstlane(v.seer_end - i0 < seers ? v.seer_end - i0 : seers);
         unsigned beg = 0, width = width0;
         while (beg < n_active) {
             member = active && lane - beg < width;
@@ -532,12 +537,30 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                             }
                 }
             } else {
-                for (int x = lo[0]; x <= hi[0]; ++x)
-                    for (int y = lo[1]; y <= hi[1]; ++y)
-                        for (int z = lo[2]; z <= hi[2]; ++z) {
-                            const TayCellRange cr = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
-                            add_range(cr.beg, cr.end);
-                        }
+                /* The group's cells in x, y, z order (z fastest), their
+                 * ranges loaded 64 at a time, one cell per lane, into the
+                 * wavefront's LDS slots, so a sweep waits for one load per
+                 * 64 cells instead of one per cell; the loop reads them back
+                 * in order. */
+                const bool empty = hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2];
+                const unsigned nz = (unsigned)(hi[2] - lo[2] + 1), nyz = nz * (unsigned)(hi[1] - lo[1] + 1);
+                const unsigned swept = empty ? 0u : nyz * (unsigned)(hi[0] - lo[0] + 1);
+                for (unsigned c0 = 0; c0 < swept; c0 += 64u) {
+                    const unsigned c = c0 + lane;
+                    if (c < swept) {
+                        const unsigned x = (unsigned)lo[0] + c / nyz, rem = c % nyz;
+                        const unsigned y = (unsigned)lo[1] + rem / nz, z = (unsigned)lo[2] + rem % nz;
+                        const TayCellRange cr = v.ranges[x + y * cx + z * cxy];
+                        cell_ranges[wave][lane][0] = cr.beg;
+                        cell_ranges[wave][lane][1] = cr.end;
+                    }
+                    tay_wave_sync();
+                    const unsigned n = swept - c0 < 64u ? swept - c0 : 64u;
+                    for (unsigned k = 0; k < n; ++k)
+                        add_range(__builtin_amdgcn_readfirstlane(cell_ranges[wave][k][0]),
+                                  __builtin_amdgcn_readfirstlane(cell_ranges[wave][k][1]));
+                    tay_wave_sync();
+                }
                 if (fill)
                     process();
             }
@@ -596,7 +619,8 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     for (int k = 0; k < 4; ++k)
         v.radii[k] = args.radii[k];
     const unsigned pairs = tay_see_tile<T, SEER_WORDS, SEEN_WORDS, NARROW, SKIP_SELF, SEEN_GRID, DIMS, COUNT,
-                                        CTX_WORDS>(v, tile * 64u, args.width, args.lane_walk, call);
+                                        CTX_WORDS>(v, tile * args.tile_seers, args.tile_seers, args.width,
+                                                   args.lane_walk, call);
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);
 }
@@ -640,7 +664,7 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
         for (int d = 0; d < 4; ++d)
             v.radii[d] = tay_uniform(args.radii[4u * b + d]);
         pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS>(
-            v, tile * 64u, args.width, args.lane_walk, call);
+            v, tile * 64u, 64u, args.width, args.lane_walk, call);
         unsigned q = 0;
         if ((threadIdx.x & 63u) == 0u)
             q = atomicAdd(args.queue, 1u);
