@@ -49,6 +49,10 @@ int tay_set_tile_width(TayState *state, unsigned width);
  * tay_gpu_program_set_lane_walk; results do not depend on it. */
 int tay_set_lane_walk(TayState *state, unsigned width);
 
+/* Seers per see tile (0 = by the seer count, the default; 64, 32 or 16),
+ * as tay_gpu_program_set_tile_seers; results do not depend on it. */
+int tay_set_tile_seers(TayState *state, unsigned seers);
+
 /* The device space of a group, valid while the simulation runs (NULL before
  * the first tay_run and for groups not on the GPU). */
 TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group);

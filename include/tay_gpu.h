@@ -208,6 +208,13 @@ int tay_gpu_program_set_tile_width(TayGpuProgram *program, unsigned width);
  * counterpart. */
 int tay_gpu_program_set_lane_walk(TayGpuProgram *program, unsigned width);
 
+/* Seers per see tile (one wavefront owns a tile): 64, 32 or 16, or 0 (the
+ * default) to choose by the pass's seer count: 64 while that gives every
+ * SIMD two wavefronts, fewer for smaller states, whose passes are bound by
+ * the sweep of their few tiles.  Results do not depend on it.  No
+ * reference counterpart. */
+int tay_gpu_program_set_tile_seers(TayGpuProgram *program, unsigned seers);
+
 /* Pairs that passed the box test (agent_see calls) since the last reset;
  * needs count_pairs.  The reference's narrow-phase telemetry counter
  * (tay/space.c:57-60). */

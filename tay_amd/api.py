@@ -111,6 +111,7 @@ _GPU = {
     "tay_get_gpu_space": (_vp, [_vp, _vp]),
     "tay_set_tile_width": (_i, [_vp, _u]),
     "tay_set_lane_walk": (_i, [_vp, _u]),
+    "tay_set_tile_seers": (_i, [_vp, _u]),
     "tay_set_order_bits": (_i, [_vp, _vp, _i]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),
@@ -147,6 +148,7 @@ _GPU = {
     "tay_gpu_program_pass_time": (_i, [_vp, _u, C.POINTER(C.c_double), C.POINTER(C.c_uint)]),
     "tay_gpu_program_set_tile_width": (_i, [_vp, _u]),
     "tay_gpu_program_set_lane_walk": (_i, [_vp, _u]),
+    "tay_gpu_program_set_tile_seers": (_i, [_vp, _u]),
     "tay_gpu_batch_create": (_vp, [C.POINTER(C.c_char_p), _u, C.POINTER(TayGpuBatchPass), _u, _u, _i, _u,
                                    C.POINTER(C.c_uint), C.POINTER(C.c_ulonglong), _i, C.c_char_p, _u]),
     "tay_gpu_batch_destroy": (None, [_vp]),

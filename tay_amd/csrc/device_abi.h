@@ -46,9 +46,10 @@ struct TaySeeArgs {
     const void *context;
     float radii[4];
     unsigned long long *pair_count;     /* pairs passing the box test, when counting */
-    unsigned tiles;                     /* number of 64-seer tiles */
+    unsigned tiles;                     /* number of tiles (tile_seers seers each) */
     unsigned width;                     /* seers per lane group at the start of a tile: 64, 32, 16 or 8 */
     unsigned lane_walk;                 /* sparse groups of at most this many lanes walk per lane */
+    unsigned tile_seers;                /* seers per tile (one wavefront): 64, 32 or 16 */
 };
 
 struct TayActArgs {

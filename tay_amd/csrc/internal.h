@@ -113,6 +113,7 @@ struct TayGpuProgram {
     int timing;
     unsigned width;             /* seers per lane group at the start of a see tile */
     unsigned lane_walk;         /* sparse lane groups up to this width walk per lane */
+    unsigned tile_seers;        /* seers per see tile: 64, 32 or 16; 0 chooses by the seer count */
 };
 
 #endif

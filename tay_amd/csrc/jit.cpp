@@ -176,6 +176,29 @@ int see_waves_override() {
     return -1;
 }
 
+/* Seers per see tile for a pass over `seers` seers.  One wavefront owns a
+ * tile and every seer meets its pairs on one lane, so a small state's
+ * 64-seer tiles leave most SIMDs without a wavefront and the pass takes as
+ * long as one tile's sweep; smaller tiles spread the state over more
+ * wavefronts (each tile's box, and so its sweep, also shrinks) at the price
+ * of idle lanes in the compact loop.  64 while that still gives every SIMD
+ * two wavefronts, else the largest of 32 and 16 that does, else 16. */
+unsigned auto_tile_seers(unsigned seers) {
+    static unsigned simds = 0;
+    if (!simds) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            simds = 4u * (unsigned)cus;
+        else
+            simds = 1024u;
+    }
+    for (unsigned s = 64u; s > 16u; s >>= 1)
+        if ((seers + s - 1u) / s >= 2u * simds)
+            return s;
+    return 16u;
+}
+
 std::string see_occupancy(int n) {
     if (n >= 1 && n <= 8)
         return " __attribute__((amdgpu_waves_per_eu(" + std::to_string(n) + ")))";
@@ -389,6 +412,7 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
     p->timing = 0;
     p->width = 64;
     p->lane_walk = TAY_GPU_DEFAULT_LANE_WALK;
+    p->tile_seers = 0;
     if (!hip_ok(hipModuleLoadData(&p->module, code.data()), "hipModuleLoadData")) {
         tay_gpu_program_destroy(p);
         return nullptr;
@@ -533,8 +557,9 @@ TAY_API int tay_gpu_program_run_pass(TayGpuProgram *p, unsigned pass_index) {
         for (int k = 0; k < 4; ++k)
             see.radii[k] = d.radii[k];
         see.pair_count = pass.pair_count;
-        see.tiles = (a->count + 63u) / 64u;
-        see.width = p->width;
+        see.tile_seers = p->tile_seers ? p->tile_seers : auto_tile_seers(a->count);
+        see.tiles = (a->count + see.tile_seers - 1u) / see.tile_seers;
+        see.width = p->width < see.tile_seers ? p->width : see.tile_seers;
         see.lane_walk = p->lane_walk;
         grid = (see.tiles + (kSeeBlock / 64u) - 1u) / (kSeeBlock / 64u);
         params[0] = &see;
@@ -576,6 +601,15 @@ TAY_API int tay_gpu_program_set_tile_width(TayGpuProgram *p, unsigned width) {
         return 0;
     }
     p->width = width;
+    return 1;
+}
+
+TAY_API int tay_gpu_program_set_tile_seers(TayGpuProgram *p, unsigned seers) {
+    if (seers != 0 && seers != 64 && seers != 32 && seers != 16) {
+        set_error("seers per tile %u: must be 0 (by the seer count), 64, 32 or 16", seers);
+        return 0;
+    }
+    p->tile_seers = seers;
     return 1;
 }
 

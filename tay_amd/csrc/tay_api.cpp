@@ -81,6 +81,7 @@ struct TayState {
     int count_pairs;
     unsigned tile_width;
     unsigned lane_walk;
+    unsigned tile_seers;        /* 0: chosen by the seer count */
     int timing;
     int fetch_mode;
 };
@@ -276,6 +277,7 @@ int state_compile(TayState *state) {
     if (state->timing)
         tay_gpu_program_enable_timing(state->program, 1);
     tay_gpu_program_set_tile_width(state->program, state->tile_width);
+    tay_gpu_program_set_tile_seers(state->program, state->tile_seers);
     if (state->lane_walk != kLibraryDefault)
         tay_gpu_program_set_lane_walk(state->program, state->lane_walk);
     return 1;
@@ -299,6 +301,7 @@ TAY_API TayState *tay_create_state() {
     state->program = nullptr;
     state->count_pairs = 0;
     state->tile_width = 64;
+    state->tile_seers = 0;
     state->lane_walk = kLibraryDefault;
     state->timing = 0;
     state->fetch_mode = TAY_FETCH_EVERY_RUN;
@@ -651,6 +654,17 @@ TAY_API int tay_set_lane_walk(TayState *state, unsigned width) {
     state->lane_walk = width;
     if (state->program)
         tay_gpu_program_set_lane_walk(state->program, width);
+    return 1;
+}
+
+TAY_API int tay_set_tile_seers(TayState *state, unsigned seers) {
+    if (seers != 0 && seers != 64 && seers != 32 && seers != 16) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "seers per tile must be 0, 64, 32 or 16");
+        return 0;
+    }
+    state->tile_seers = seers;
+    if (state->program)
+        tay_gpu_program_set_tile_seers(state->program, seers);
     return 1;
 }
 

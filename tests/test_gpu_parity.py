@@ -273,6 +273,29 @@ def test_lane_walk_bit_exact(gpu, oracle, models, walk):
     assert_bitwise(out[0], out[1])
 
 
+def test_tile_seers_bit_exact(gpu, oracle, models):
+    """Tiles of 64, 32 or 16 seers (and the choice by seer count), with
+    narrower starting lane groups and without the lane walk: every seer
+    still meets its pairs in the reference's order, bit-exact with the
+    oracle, on a clustered flocking state (dense flocks and sparse
+    outskirts) and on the uniform 10k-agent plain workload."""
+    cases = [(W.flock_model(25.0), W.flock_agents(20000, seed=67, clusters=30, sigma=20.0, box=1500.0), 3),
+             (W.plain_model(40.0), W.plain_agents(10000, seed=68), 2)]
+    for model, recs, steps in cases:
+        with W.Simulation(oracle, model, recs, api.TAY_GPU_Z_GRID, funcs=models) as s:
+            s.run(steps)
+            want = s.agents()
+        for seers, width, walk in ((0, 64, 64), (64, 64, 64), (32, 64, 64), (16, 64, 64), (16, 8, 64), (32, 16, 0)):
+            print("case: %s seers=%d width=%d walk=%d" % (model.passes[0].func, seers, width, walk), flush=True)
+            with W.Simulation(gpu, model, recs, api.TAY_GPU_Z_GRID) as s:
+                assert gpu.tay_set_tile_seers(s.state, seers)
+                assert gpu.tay_set_tile_width(s.state, width)
+                assert gpu.tay_set_lane_walk(s.state, walk)
+                s.run(steps)
+                assert_bitwise(s.agents(), want)
+                assert not gpu.tay_set_tile_seers(s.state, 48)
+
+
 def test_device_sqrt_is_ieee_for_every_float(gpu):
     """The square root model code gets on the device (sqrt/sqrtf of a float)
     equals the IEEE square root bit for bit on all 2^32 float inputs, both
