@@ -88,17 +88,17 @@ B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
 # profiles/see_flock1M_pmc.md).  A counter run cannot be made from inside
 # this process: the figure is carried here, with the commit of the kernel it
 # was measured on, and re-measured whenever the see kernel changes.
-SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of commit d03ca60",
-               "bytes": int((39536.6 * 2.000 + 42968.8 * 1.000) * 1024)}
+SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of commit c7e55b7",
+               "bytes": int((43705.6 * 2.000 + 54675.3 * 1.000) * 1024)}
 # The SPH line's see passes on 2M particles (profiles/see_sph2M_pmc.md),
 # same calibration.  Algorithmic bytes per particle: forces, the seer reads
 # p, v, pressure and both accumulators (15 words) and writes the
 # accumulators (8), the seen side reads p, v, density and pressure (8);
 # density, the seer reads p and density (4) and writes density (1), the seen
 # side reads p (3).
-SPH_TRAFFIC = {"particles": 2_000_000, "kernel": "see kernels of commit d03ca60",
-               "forces": int((72749.3 * 2.000 + 46875.0 * 1.000) * 1024),
-               "density": int((19941.5 * 2.000 + 7812.5 * 1.000) * 1024)}
+SPH_TRAFFIC = {"particles": 2_000_000, "kernel": "see kernels of commit c7e55b7",
+               "forces": int((69859.7 * 2.000 + 46875.0 * 1.000) * 1024),
+               "density": int((19950.6 * 2.000 + 7812.5 * 1.000) * 1024)}
 B_SPH_FORCES_PER_PARTICLE = 4 * (15 + 8 + 8)
 B_SPH_DENSITY_PER_PARTICLE = 4 * (4 + 1 + 3)
 
