@@ -444,21 +444,30 @@ def bench_ensemble(lib, args):
     from tay_amd import workloads as W
     ens = W.ensemble(args.ensemble_instances, seed=7)
     radius = [m.passes[0].radius for _, m, _ in ens]
-    factors, widths, walks = (1.0, 1.5, 2.0), (64, 32), (0, 16, 64)
-    # tables sized for the smallest cell factor, shared by every run below
-    tables = [W.cell_table_mb(side, min(factors) * r) for (_, _, side), r in zip(ens, radius)]
+    # cell sizes per instance by density: about `target` agents per cell,
+    # between 1 and 4 radii (dense instances get fine grids, sparse ones
+    # coarse grids; below one radius a seer's box spans three or more cells
+    # per axis)
+    targets, fmin, fmax = (4.0, 8.0, 16.0), 1.0, 4.0
+    widths, walks = (64, 32), (16, 64)
+    # tables sized for the smallest target's cells, shared by every run below
+    tables = [W.cell_table_mb(side, W.density_part(rec, r, min(targets), fmin, fmax))
+              for (rec, _, side), r in zip(ens, radius)]
     steps, warm = args.ensemble_steps, 2
 
+    parts = np.zeros((len(ens), 4), np.float32)
     with W.Batch(lib, ens, radius, tables) as b:
-        f = (ctypes.c_float * len(factors))(*factors)
+        tg = (ctypes.c_float * len(targets))(*targets)
         w = (ctypes.c_uint * len(widths))(*widths)
         lw = (ctypes.c_uint * len(walks))(*walks)
-        table = (ctypes.c_double * (len(factors) * len(widths) * len(walks)))()
-        bf, bw, bl = ctypes.c_float(), ctypes.c_uint(), ctypes.c_uint()
-        if not lib.tay_gpu_batch_autotune(b.batch, f, len(factors), w, len(widths), lw, len(walks), 3, table,
-                                          ctypes.byref(bf), ctypes.byref(bw), ctypes.byref(bl)):
+        table = (ctypes.c_double * (len(targets) * len(widths) * len(walks)))()
+        bt, bw, bl = ctypes.c_float(), ctypes.c_uint(), ctypes.c_uint()
+        if not lib.tay_gpu_batch_autotune_density(b.batch, tg, len(targets), fmin, fmax, w, len(widths), lw,
+                                                  len(walks), 3, table, ctypes.byref(bt),
+                                                  parts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                  ctypes.byref(bw), ctypes.byref(bl)):
             raise SystemExit("autotune: " + lib.tay_gpu_last_error().decode())
-    part = [bf.value * r for r in radius]
+    part = [float(x) for x in parts[:, 0]]
     width, walk = bw.value, bl.value
 
     with W.Batch(lib, ens, part, tables, count_pairs=True) as twin:
@@ -521,10 +530,14 @@ def bench_ensemble(lib, args):
                          "ms_per_step": inst_wall * 1e3 / steps, "see_ms_per_step": inst_see_ms / steps,
                          "launches_per_pass_per_step": len(ens)},
         "identical": bool(identical),
-        "autotune": {"cell_factors": list(factors), "tile_widths": list(widths), "lane_walks": list(walks),
+        "autotune": {"agents_per_cell": list(targets), "cell_factor_bounds": [fmin, fmax],
+                     "tile_widths": list(widths), "lane_walks": list(walks),
                      "ms_per_step": [[[table[(i * len(widths) + j) * len(walks) + k] for k in range(len(walks))]
-                                      for j in range(len(widths))] for i in range(len(factors))],
-                     "chosen": {"cell_factor": bf.value, "tile_width": width, "lane_walk": walk}},
+                                      for j in range(len(widths))] for i in range(len(targets))],
+                     "chosen": {"agents_per_cell": bt.value, "tile_width": width, "lane_walk": walk,
+                                "cell_factors": {"min": float(min(p / r for p, r in zip(part, radius))),
+                                                 "median": float(np.median([p / r for p, r in zip(part, radius)])),
+                                                 "max": float(max(p / r for p, r in zip(part, radius)))}}},
         "roofline": roofline(p_see / steps * f_pair, see_ms / 1e3 / launches, "batched see (tay_batch_pass_0)",
                              f_pair, "f_pair = the operations every box-passing pair of flock_see executes (box "
                                      "test and the always-taken part), a lower bound"),

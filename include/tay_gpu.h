@@ -266,6 +266,15 @@ int tay_gpu_batch_compile_check(const char *const *sources, unsigned sources_cou
 
 /* instances x 4 floats: suggested cell sizes (min_part_sizes) per instance */
 int tay_gpu_batch_set_part_sizes(TayGpuBatch *batch, const float *part_sizes);
+/* instances x 4 floats: the part sizes in use (1 until set) */
+int tay_gpu_batch_get_part_sizes(const TayGpuBatch *batch, float *part_sizes);
+/* instances x 4 floats: part sizes for about `target` agents per cell.  An
+ * instance of n agents whose current bounding box has volume V (area or
+ * length in fewer dimensions) gets cells of side (target V / n)^(1/dims),
+ * clamped per axis to [min_factor, max_factor] times its radius of the
+ * first see pass.  Computes them without applying them. */
+int tay_gpu_batch_density_part_sizes(TayGpuBatch *batch, float target, float min_factor, float max_factor,
+                                     float *part_sizes);
 /* instances x 4 floats: the see pass's radii per instance */
 int tay_gpu_batch_set_radii(TayGpuBatch *batch, unsigned pass_index, const float *radii);
 /* instances x context_size bytes, packed: the pass's context per instance */
@@ -318,6 +327,18 @@ int tay_gpu_batch_autotune(TayGpuBatch *batch, const float *cell_factors, unsign
                            const unsigned *widths, unsigned widths_count, const unsigned *lane_walks,
                            unsigned lane_walks_count, unsigned steps, double *table_ms, float *best_factor,
                            unsigned *best_width, unsigned *best_lane_walk);
+
+/* As tay_gpu_batch_autotune, over agents-per-cell targets instead of one
+ * cell factor for every instance: each target gives every instance its own
+ * cell size by its density (tay_gpu_batch_density_part_sizes, densities of
+ * the starting agents), so dense and sparse instances of one ensemble get
+ * fine and coarse grids.  Applies the fastest combination and writes the
+ * chosen target and (optional, instances x 4 floats) its part sizes. */
+int tay_gpu_batch_autotune_density(TayGpuBatch *batch, const float *targets, unsigned targets_count,
+                                   float min_factor, float max_factor, const unsigned *widths, unsigned widths_count,
+                                   const unsigned *lane_walks, unsigned lane_walks_count, unsigned steps,
+                                   double *table_ms, float *best_target, float *part_sizes, unsigned *best_width,
+                                   unsigned *best_lane_walk);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,8 @@ _GPU = {
     "tay_gpu_batch_compile_check": (_i, [C.POINTER(C.c_char_p), _u, C.POINTER(TayGpuBatchPass), _u, _u, _i,
                                          C.c_char_p, _u, C.POINTER(C.c_ulonglong)]),
     "tay_gpu_batch_set_part_sizes": (_i, [_vp, C.POINTER(C.c_float)]),
+    "tay_gpu_batch_get_part_sizes": (_i, [_vp, C.POINTER(C.c_float)]),
+    "tay_gpu_batch_density_part_sizes": (_i, [_vp, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]),
     "tay_gpu_batch_set_radii": (_i, [_vp, _u, C.POINTER(C.c_float)]),
     "tay_gpu_batch_set_contexts": (_i, [_vp, _u, _vp]),
     "tay_gpu_batch_set_tile_width": (_i, [_vp, _u]),
@@ -178,6 +180,10 @@ _GPU = {
     "tay_gpu_batch_autotune": (_i, [_vp, C.POINTER(C.c_float), _u, C.POINTER(C.c_uint), _u, C.POINTER(C.c_uint), _u,
                                     _u, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint),
                                     C.POINTER(C.c_uint)]),
+    "tay_gpu_batch_autotune_density": (_i, [_vp, C.POINTER(C.c_float), _u, C.c_float, C.c_float,
+                                            C.POINTER(C.c_uint), _u, C.POINTER(C.c_uint), _u, _u,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                            C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
 }
 
 # test-only inspection calls of the oracle (oracle/oracle.h)

@@ -70,12 +70,13 @@ def _run(cmd, cwd=None):
 
 def build_library(verbose=False, out=LIB, overrides=None, build_dir=BUILD):
     """Compile every HIP/C++ source for gfx950 and link libtay_amd.so (or
-    `out`, embedding the `overrides` headers, for experiments)."""
+    `out`, for experiments, with the embedded headers or sources named in
+    `overrides` taken from other files)."""
     embedded = _write_embedded(overrides, build_dir)
     objs = []
     jobs = []
     for src in SOURCES + [embedded]:
-        path = src if os.path.isabs(src) else os.path.join(CSRC, src)
+        path = src if os.path.isabs(src) else (overrides or {}).get(src, os.path.join(CSRC, src))
         obj = os.path.join(build_dir, os.path.basename(path) + ".o")
         objs.append(obj)
         if src in HOST_ONLY:

@@ -188,6 +188,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_cost(const unsigned *__res
                                                            const TayBatchInstance *__restrict__ inst, int dims,
                                                            const TayGridGeom *__restrict__ geom,
                                                            const TayCellRange *__restrict__ ranges,
+                                                           const float *__restrict__ radii,
                                                            unsigned *__restrict__ tile_bucket,
                                                            unsigned *__restrict__ bucket_hist) {
     __shared__ unsigned h[kCostBuckets];
@@ -202,7 +203,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_cost(const unsigned *__res
         const unsigned n = in.off + in.count - s0 < 64u ? in.off + in.count - s0 : 64u;
         const unsigned mid = s0 + n / 2u;
         const TayCellRange r = ranges[in.ranges_off + key_to_cell(keys[mid], dims, geom + b)];
-        const unsigned cost = (r.end - r.beg) * n;
+        /* candidates per seer: the cell's agents times the cells a seer's
+         * box spans, so instances with different cell sizes compare */
+        float span = 1.0f;
+        if (radii)
+            for (int k = 0; k < dims; ++k)
+                span *= 2.0f * radii[4u * b + (unsigned)k] / geom[b].cell_size[k] + 1.0f;
+        const float c = (float)(r.end - r.beg) * (float)n * span;
+        const unsigned cost = c < 4.0e9f ? (unsigned)c : 0xFFFFFFFFu;
         const unsigned bucket = cost ? 31u - (unsigned)__clz(cost) : 0u;
         tile_bucket[t] = bucket;
         atomicAdd(&h[bucket], 1u);
@@ -329,6 +337,7 @@ struct TayGpuBatch {
     unsigned sort_tiles;        /* radix tiles of the per-instance segmented sort */
     std::vector<TayBatchInstance> inst_host;
     std::vector<float> radius_host;     /* first see pass's radii, 4 per instance */
+    std::vector<float> parts_host;      /* part sizes, 4 per instance */
     unsigned long long total_cells;
     unsigned long long max_instance_cells;
 
@@ -435,8 +444,12 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
                                                          b->inst, b->dims, b->geom, b->ranges, src, dst);
     const unsigned tile_blocks = blocks_for(b->tiles, kBuildBlock, 0xFFFFFFFFu);
     if (b->order_mode == TAY_GPU_TILE_ORDER_COST) {
+        const float *radii = nullptr;
+        for (const TayGpuBatch::Pass &p : b->passes)
+            if (!radii && p.desc.type == TAY_GPU_PASS_SEE)
+                radii = p.radii;
         k_tile_cost<<<tile_blocks, kBuildBlock, 0, s>>>(b->keys[f], b->tiles, b->tile_inst, b->inst, b->dims,
-                                                        b->geom, b->ranges, b->tile_bucket, b->bucket_hist);
+                                                        b->geom, b->ranges, radii, b->tile_bucket, b->bucket_hist);
         k_tile_scan<<<1, 64, 0, s>>>(b->bucket_hist, b->bucket_off, b->queues, (unsigned)b->passes.size());
         k_tile_scatter<<<tile_blocks, kBuildBlock, 0, s>>>(b->tile_bucket, b->tiles, b->bucket_off, b->order);
     } else {
@@ -657,6 +670,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
         for (unsigned t = inst[i].off / 64u; t < (inst[i].off + inst[i].count + 63u) / 64u; ++t)
             tile_inst[t] = i;
     std::vector<float> parts(4ull * instances, 1.0f);
+    b->parts_host = parts;
     bool ok = device_alloc((void **)&b->cols[0], col_bytes, "batch columns") &&
               device_alloc((void **)&b->cols[1], col_bytes, "batch columns") &&
               hip_ok(hipMemsetAsync(b->cols[0], 0, col_bytes, s), "hipMemsetAsync") &&
@@ -771,9 +785,17 @@ TAY_API int tay_gpu_batch_set_part_sizes(TayGpuBatch *b, const float *part_sizes
                 return 0;
             }
     hipStream_t s = stream();
-    return s && hip_ok(hipMemcpyAsync(b->parts, part_sizes, 16ull * b->instances, hipMemcpyHostToDevice, s),
-                       "upload part sizes") &&
-           hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (!(s && hip_ok(hipMemcpyAsync(b->parts, part_sizes, 16ull * b->instances, hipMemcpyHostToDevice, s),
+                      "upload part sizes") &&
+          hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize")))
+        return 0;
+    b->parts_host.assign(part_sizes, part_sizes + 4ull * b->instances);
+    return 1;
+}
+
+TAY_API int tay_gpu_batch_get_part_sizes(const TayGpuBatch *b, float *part_sizes) {
+    std::memcpy(part_sizes, b->parts_host.data(), 16ull * b->instances);
+    return 1;
 }
 
 TAY_API int tay_gpu_batch_set_radii(TayGpuBatch *b, unsigned pass_index, const float *radii) {
@@ -985,53 +1007,107 @@ TAY_API int tay_gpu_batch_reset_pair_counts(TayGpuBatch *b) {
     return 1;
 }
 
-TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, unsigned factors_count,
-                                   const unsigned *widths, unsigned widths_count, const unsigned *lane_walks,
-                                   unsigned lane_walks_count, unsigned steps, double *table_ms, float *best_factor,
-                                   unsigned *best_width, unsigned *best_lane_walk) {
-    if (!factors_count || !widths_count || !lane_walks_count || !steps) {
-        set_error("autotune needs at least one cell factor, tile width, lane-walk width and step");
-        return 0;
+namespace {
+
+bool radii_set(const TayGpuBatch *b, const char *what) {
+    for (unsigned i = 0; i < b->instances; ++i)
+        for (int k = 0; k < b->dims; ++k)
+            if (!(b->radius_host[4ull * i + k] > 0.0f)) {
+                set_error("%s: instance %u has no positive radius on axis %d (set the first see pass's radii)", what,
+                          i, k);
+                return false;
+            }
+    return true;
+}
+
+bool tune_widths_ok(const unsigned *widths, unsigned widths_count, const unsigned *lane_walks,
+                    unsigned lane_walks_count, unsigned steps) {
+    if (!widths_count || !lane_walks_count || !steps) {
+        set_error("autotune needs at least one tile width, lane-walk width and step");
+        return false;
     }
     for (unsigned j = 0; j < lane_walks_count; ++j)
         if (lane_walks[j] > 64) {
             set_error("lane walk width %u: must be at most 64", lane_walks[j]);
-            return 0;
+            return false;
         }
     for (unsigned j = 0; j < widths_count; ++j)
         if (widths[j] != 64 && widths[j] != 32 && widths[j] != 16 && widths[j] != 8) {
             set_error("tile width %u: must be 64, 32, 16 or 8", widths[j]);
-            return 0;
+            return false;
         }
-    for (unsigned i = 0; i < factors_count; ++i)
-        if (!(cell_factors[i] > 0.0f)) {
-            set_error("cell factor %u must be positive", i);
-            return 0;
-        }
-    for (unsigned i = 0; i < b->instances; ++i)
+    return true;
+}
+
+/* Every instance's geometry, and so the bounding box of its agents, from
+ * the current columns and part sizes. */
+bool instance_geometry(TayGpuBatch *b, std::vector<TayGridGeom> &geom) {
+    hipStream_t s = stream();
+    if (!s)
+        return false;
+    if (b->tiles)
+        k_tile_box<<<(b->tiles + kBuildBlock / 64u - 1u) / (kBuildBlock / 64u), kBuildBlock, 0, s>>>(
+            b->cols[b->cur], b->slots, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes);
+    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom);
+    geom.resize(b->instances);
+    return launch_ok("instance geometry") &&
+           hip_ok(hipMemcpyAsync(geom.data(), b->geom, sizeof(TayGridGeom) * b->instances, hipMemcpyDeviceToHost,
+                                 s),
+                  "download geometry") &&
+           hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+/* Cell sizes for `target` agents per cell: an instance of n agents whose
+ * bounding box has volume V (area, length in fewer dimensions) gets cells of
+ * side (target V / n)^(1/dims), clamped per axis to [min_factor, max_factor]
+ * times its radius. */
+void density_parts(const TayGpuBatch *b, const std::vector<TayGridGeom> &geom, float target, float min_factor,
+                   float max_factor, float *parts) {
+    for (unsigned i = 0; i < b->instances; ++i) {
+        double vol = 1.0;
         for (int k = 0; k < b->dims; ++k)
-            if (!(b->radius_host[4ull * i + k] > 0.0f)) {
-                set_error("autotune: instance %u has no positive radius on axis %d (set the first see pass's radii)",
-                          i, k);
-                return 0;
+            vol *= std::max(0.0, (double)geom[i].box_max[k] - (double)geom[i].box_min[k]);
+        const unsigned n = b->inst_host[i].count;
+        const double side = n && vol > 0.0 ? std::pow((double)target * vol / n, 1.0 / b->dims) : 0.0;
+        for (int k = 0; k < 4; ++k) {
+            float part = 1.0f;
+            if (k < b->dims) {
+                const double r = b->radius_host[4ull * i + k];
+                part = (float)std::min(std::max(side, (double)min_factor * r), (double)max_factor * r);
             }
+            parts[4ull * i + k] = part;
+        }
+    }
+}
+
+bool density_args_ok(float min_factor, float max_factor) {
+    if (!(min_factor > 0.0f && min_factor <= max_factor && std::isfinite(max_factor))) {
+        set_error("cell factor bounds [%g, %g]: need 0 < min <= max", min_factor, max_factor);
+        return false;
+    }
+    return true;
+}
+
+/* Times every candidate set of part sizes (4 per instance each) with every
+ * tile width and lane-walk width, restoring the starting agents before each
+ * run of `steps` un-captured steps; table_ms (optional, candidates x widths x
+ * lane walks, lane walk fastest) receives ms per step.  Applies the fastest
+ * combination and leaves the agents as they were. */
+bool autotune_parts(TayGpuBatch *b, const std::vector<std::vector<float>> &cands, const unsigned *widths,
+                    unsigned widths_count, const unsigned *lane_walks, unsigned lane_walks_count, unsigned steps,
+                    double *table_ms, unsigned *best_cand, unsigned *best_width, unsigned *best_lane_walk) {
     hipStream_t s = stream();
     const size_t col_bytes = (size_t)b->words * b->slots * 4;
     if (!s || (!b->backup && !device_alloc((void **)&b->backup, col_bytes, "autotune backup")))
-        return 0;
+        return false;
     const int start_cur = b->cur;
     const int timing = b->timing;
     b->timing = 0;
     bool ok = hip_ok(hipMemcpyAsync(b->backup, b->cols[start_cur], col_bytes, hipMemcpyDeviceToDevice, s), "backup");
     double best = 1e300;
-    float bf = cell_factors[0];
-    unsigned bw = widths[0], bl = lane_walks[0];
-    std::vector<float> parts(4ull * b->instances, 1.0f);
-    for (unsigned i = 0; ok && i < factors_count; ++i) {
-        for (unsigned k = 0; k < b->instances; ++k)
-            for (int d = 0; d < b->dims; ++d)
-                parts[4ull * k + d] = cell_factors[i] * b->radius_host[4ull * k + d];
-        ok = tay_gpu_batch_set_part_sizes(b, parts.data());
+    unsigned bc = 0, bw = widths[0], bl = lane_walks[0];
+    for (unsigned i = 0; ok && i < cands.size(); ++i) {
+        ok = tay_gpu_batch_set_part_sizes(b, cands[i].data());
         for (unsigned j = 0; ok && j < widths_count; ++j)
             for (unsigned l = 0; ok && l < lane_walks_count; ++l) {
                 b->width = widths[j];
@@ -1046,7 +1122,7 @@ TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, un
                     table_ms[((size_t)i * widths_count + j) * lane_walks_count + l] = ms;
                 if (ok && ms < best) {
                     best = ms;
-                    bf = cell_factors[i];
+                    bc = i;
                     bw = widths[j];
                     bl = lane_walks[l];
                 }
@@ -1054,19 +1130,93 @@ TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, un
     }
     b->cur = start_cur;
     ok = ok && hip_ok(hipMemcpyAsync(b->cols[start_cur], b->backup, col_bytes, hipMemcpyDeviceToDevice, s), "restore");
-    for (unsigned k = 0; k < b->instances; ++k)
-        for (int d = 0; d < b->dims; ++d)
-            parts[4ull * k + d] = bf * b->radius_host[4ull * k + d];
-    ok = ok && tay_gpu_batch_set_part_sizes(b, parts.data()) && tay_gpu_batch_reset_pair_counts(b);
+    ok = ok && tay_gpu_batch_set_part_sizes(b, cands[bc].data()) && tay_gpu_batch_reset_pair_counts(b);
     drop_graph(b);
     b->width = bw;
     b->lane_walk = bl;
     b->timing = timing;
-    if (best_factor)
-        *best_factor = bf;
+    *best_cand = bc;
     if (best_width)
         *best_width = bw;
     if (best_lane_walk)
         *best_lane_walk = bl;
+    return ok;
+}
+
+}  // namespace
+
+TAY_API int tay_gpu_batch_autotune(TayGpuBatch *b, const float *cell_factors, unsigned factors_count,
+                                   const unsigned *widths, unsigned widths_count, const unsigned *lane_walks,
+                                   unsigned lane_walks_count, unsigned steps, double *table_ms, float *best_factor,
+                                   unsigned *best_width, unsigned *best_lane_walk) {
+    if (!factors_count) {
+        set_error("autotune needs at least one cell factor, tile width, lane-walk width and step");
+        return 0;
+    }
+    if (!tune_widths_ok(widths, widths_count, lane_walks, lane_walks_count, steps))
+        return 0;
+    for (unsigned i = 0; i < factors_count; ++i)
+        if (!(cell_factors[i] > 0.0f)) {
+            set_error("cell factor %u must be positive", i);
+            return 0;
+        }
+    if (!radii_set(b, "autotune"))
+        return 0;
+    std::vector<std::vector<float>> cands(factors_count, std::vector<float>(4ull * b->instances, 1.0f));
+    for (unsigned i = 0; i < factors_count; ++i)
+        for (unsigned k = 0; k < b->instances; ++k)
+            for (int d = 0; d < b->dims; ++d)
+                cands[i][4ull * k + d] = cell_factors[i] * b->radius_host[4ull * k + d];
+    unsigned bc = 0;
+    const bool ok = autotune_parts(b, cands, widths, widths_count, lane_walks, lane_walks_count, steps, table_ms, &bc,
+                                   best_width, best_lane_walk);
+    if (best_factor)
+        *best_factor = cell_factors[bc];
+    return ok;
+}
+
+TAY_API int tay_gpu_batch_density_part_sizes(TayGpuBatch *b, float target, float min_factor, float max_factor,
+                                             float *part_sizes) {
+    if (!(target > 0.0f && std::isfinite(target))) {
+        set_error("agents per cell %g: must be positive", target);
+        return 0;
+    }
+    std::vector<TayGridGeom> geom;
+    if (!density_args_ok(min_factor, max_factor) || !radii_set(b, "density part sizes") ||
+        !instance_geometry(b, geom))
+        return 0;
+    density_parts(b, geom, target, min_factor, max_factor, part_sizes);
+    return 1;
+}
+
+TAY_API int tay_gpu_batch_autotune_density(TayGpuBatch *b, const float *targets, unsigned targets_count,
+                                           float min_factor, float max_factor, const unsigned *widths,
+                                           unsigned widths_count, const unsigned *lane_walks,
+                                           unsigned lane_walks_count, unsigned steps, double *table_ms,
+                                           float *best_target, float *part_sizes, unsigned *best_width,
+                                           unsigned *best_lane_walk) {
+    if (!targets_count) {
+        set_error("autotune needs at least one agents-per-cell target");
+        return 0;
+    }
+    for (unsigned i = 0; i < targets_count; ++i)
+        if (!(targets[i] > 0.0f && std::isfinite(targets[i]))) {
+            set_error("agents per cell %u must be positive", i);
+            return 0;
+        }
+    std::vector<TayGridGeom> geom;
+    if (!tune_widths_ok(widths, widths_count, lane_walks, lane_walks_count, steps) ||
+        !density_args_ok(min_factor, max_factor) || !radii_set(b, "autotune") || !instance_geometry(b, geom))
+        return 0;
+    std::vector<std::vector<float>> cands(targets_count, std::vector<float>(4ull * b->instances, 1.0f));
+    for (unsigned i = 0; i < targets_count; ++i)
+        density_parts(b, geom, targets[i], min_factor, max_factor, cands[i].data());
+    unsigned bc = 0;
+    const bool ok = autotune_parts(b, cands, widths, widths_count, lane_walks, lane_walks_count, steps, table_ms, &bc,
+                                   best_width, best_lane_walk);
+    if (best_target)
+        *best_target = targets[bc];
+    if (part_sizes)
+        std::memcpy(part_sizes, cands[bc].data(), 16ull * b->instances);
     return ok;
 }

@@ -221,7 +221,7 @@ struct TaySeeView {
  * further (0: always split).  Returns the pairs this lane passed to the
  * model when COUNT is set. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
-          bool COUNT, unsigned CTX_WORDS, class Call>
+          bool COUNT, unsigned CTX_WORDS, bool ROW_WALK, class Call>
 __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned seers, unsigned width0,
                                                 unsigned lane_walk,
                                                 Call call) {
@@ -457,13 +457,20 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                 /* A sparse group of at most lane_walk lanes: every member
                  * walks the cells of its own box, in x, y, z order and each
                  * cell in storage order, all members at once instead of one
-                 * sub-group after another.  Memory latency is paid per batch:
-                 * the ranges of up to WALK consecutive z cells are loaded
+                 * sub-group after another.  Memory latency is paid per batch.
+                 * Plain tiles load the ranges of WALK consecutive z cells
                  * together, then each cell's candidates WALK at a time
                  * (positions loaded together, box-tested into a mask, the
-                 * model called for the set bits in ascending order). */
+                 * model called for the set bits in ascending order).  Batched
+                 * tiles (ROW_WALK), mostly sparse instances with a few agents
+                 * per cell, load the ranges of ZB z cells of a row together
+                 * and take the row's candidates RW at a time across its
+                 * cells, in the same order: fewer waits per seer, for about
+                 * forty more registers, which the batched kernel has to spare
+                 * and the plain kernels, compiled for seven or eight
+                 * wavefronts per SIMD, would spill. */
                 if (member) {
-                    constexpr unsigned WALK = 2;
+                    constexpr unsigned WALK = 2, ZB = 4, RW = 4;
                     int plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
 #pragma unroll
                     for (int k = 0; k < DIMS; ++k) {
@@ -478,25 +485,35 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                                                g->count[k]);
                         phi[k] = tay_cell_axis((ah + r[k]) + mhi, g->origin[k], g->cell_size[k], g->count[k]);
                     }
-                    for (int x = plo[0]; x <= phi[0]; ++x)
-                        for (int y = plo[1]; y <= phi[1]; ++y)
-                            for (int z0 = plo[2]; z0 <= phi[2]; z0 += (int)WALK) {
-                                TayCellRange crs[WALK];
+                    if constexpr (ROW_WALK) {
+                        for (int x = plo[0]; x <= phi[0]; ++x)
+                            for (int y = plo[1]; y <= phi[1]; ++y)
+                                for (int z0 = plo[2]; z0 <= phi[2]; z0 += (int)ZB) {
+                                    /* ranges of up to ZB cells, then the row's
+                                     * candidates as one sequence: position s lies
+                                     * in the last cell q with first[q] <= s */
+                                    unsigned beg[ZB], first[ZB], total = 0;
 #pragma unroll
-                                for (unsigned q = 0; q < WALK; ++q) {
-                                    const int z = z0 + (int)q <= phi[2] ? z0 + (int)q : phi[2];
-                                    crs[q] = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
-                                }
+                                    for (unsigned q = 0; q < ZB; ++q) {
+                                        const int z = z0 + (int)q <= phi[2] ? z0 + (int)q : phi[2];
+                                        const TayCellRange cr =
+                                            v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                                        beg[q] = cr.beg;
+                                        first[q] = total;
+                                        total += z0 + (int)q <= phi[2] ? cr.end - cr.beg : 0u;
+                                    }
+                                    for (unsigned s0 = 0; s0 < total; s0 += RW) {
+                                        unsigned idx[RW];
+                                        float b_lo[RW][3], b_hi[RW][3];
 #pragma unroll
-                                for (unsigned q = 0; q < WALK; ++q) {
-                                    if (z0 + (int)q > phi[2])
-                                        break;
-                                    const unsigned end = crs[q].end;
-                                    for (unsigned j0 = crs[q].beg; j0 < end; j0 += WALK) {
-                                        float b_lo[WALK][3], b_hi[WALK][3];
+                                        for (unsigned t = 0; t < RW; ++t) {
+                                            const unsigned s = s0 + t < total ? s0 + t : total - 1u;
+                                            unsigned j = beg[0] + s;
 #pragma unroll
-                                        for (unsigned t = 0; t < WALK; ++t) {
-                                            const unsigned off = (j0 + t < end ? j0 + t : j0) * 4u;
+                                            for (unsigned q = 1; q < ZB; ++q)
+                                                j = first[q] <= s ? beg[q] + (s - first[q]) : j;
+                                            idx[t] = j;
+                                            const unsigned off = j * 4u;
 #pragma unroll
                                             for (int k = 0; k < 3; ++k) {
                                                 b_lo[t][k] = k < DIMS ? __uint_as_float(
@@ -508,33 +525,90 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                                                                  : b_lo[t][k];
                                             }
                                         }
-                                        unsigned m = 0;
+                                        bool ok[RW];
 #pragma unroll
-                                        for (unsigned t = 0; t < WALK; ++t) {
-                                            const unsigned j = j0 + t;
-                                            bool ok = j < end && tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo[t], b_hi[t], r);
+                                        for (unsigned t = 0; t < RW; ++t) {
+                                            ok[t] = s0 + t < total &&
+                                                    tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo[t], b_hi[t], r);
                                             if (SKIP_SELF)
-                                                ok = ok && j != i;
-                                            m |= (unsigned)ok << t;
+                                                ok[t] = ok[t] && idx[t] != i;
                                         }
-                                        while (m) {
-                                            const unsigned j = j0 + (unsigned)__builtin_ctz(m);
-                                            m &= m - 1u;
-                                            const unsigned off = j * 4u;
-                                            union {
-                                                typename T::seen agent;
-                                                unsigned w[SEEN_WORDS];
-                                            } B;
 #pragma unroll
-                                            for (unsigned k = 0; k < SEEN_WORDS; ++k)
-                                                B.w[k] = *(const unsigned *)(seen_col[k] + off);
-                                            if (COUNT)
-                                                ++pairs;
-                                            call(&A.agent, &B.agent, ctx);
+                                        for (unsigned t = 0; t < RW; ++t)
+                                            if (ok[t]) {
+                                                union {
+                                                    typename T::seen agent;
+                                                    unsigned w[SEEN_WORDS];
+                                                } B;
+                                                const unsigned off = idx[t] * 4u;
+#pragma unroll
+                                                for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                                                    B.w[k] = *(const unsigned *)(seen_col[k] + off);
+                                                if (COUNT)
+                                                    ++pairs;
+                                                call(&A.agent, &B.agent, ctx);
+                                            }
+                                    }
+                                }
+                    } else {
+                        for (int x = plo[0]; x <= phi[0]; ++x)
+                            for (int y = plo[1]; y <= phi[1]; ++y)
+                                for (int z0 = plo[2]; z0 <= phi[2]; z0 += (int)WALK) {
+                                    TayCellRange crs[WALK];
+#pragma unroll
+                                    for (unsigned q = 0; q < WALK; ++q) {
+                                        const int z = z0 + (int)q <= phi[2] ? z0 + (int)q : phi[2];
+                                        crs[q] = v.ranges[(unsigned)x + (unsigned)y * cx + (unsigned)z * cxy];
+                                    }
+#pragma unroll
+                                    for (unsigned q = 0; q < WALK; ++q) {
+                                        if (z0 + (int)q > phi[2])
+                                            break;
+                                        const unsigned end = crs[q].end;
+                                        for (unsigned j0 = crs[q].beg; j0 < end; j0 += WALK) {
+                                            float b_lo[WALK][3], b_hi[WALK][3];
+#pragma unroll
+                                            for (unsigned t = 0; t < WALK; ++t) {
+                                                const unsigned off = (j0 + t < end ? j0 + t : j0) * 4u;
+#pragma unroll
+                                                for (int k = 0; k < 3; ++k) {
+                                                    b_lo[t][k] = k < DIMS ? __uint_as_float(
+                                                                                *(const unsigned *)(seen_col[2 + k] + off))
+                                                                          : 0.0f;
+                                                    b_hi[t][k] = SEEN_BOX && k < DIMS
+                                                                     ? __uint_as_float(
+                                                                           *(const unsigned *)(seen_col[6 + k] + off))
+                                                                     : b_lo[t][k];
+                                                }
+                                            }
+                                            unsigned m = 0;
+#pragma unroll
+                                            for (unsigned t = 0; t < WALK; ++t) {
+                                                const unsigned j = j0 + t;
+                                                bool ok = j < end && tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo[t], b_hi[t], r);
+                                                if (SKIP_SELF)
+                                                    ok = ok && j != i;
+                                                m |= (unsigned)ok << t;
+                                            }
+                                            while (m) {
+                                                const unsigned j = j0 + (unsigned)__builtin_ctz(m);
+                                                m &= m - 1u;
+                                                const unsigned off = j * 4u;
+                                                union {
+                                                    typename T::seen agent;
+                                                    unsigned w[SEEN_WORDS];
+                                                } B;
+#pragma unroll
+                                                for (unsigned k = 0; k < SEEN_WORDS; ++k)
+                                                    B.w[k] = *(const unsigned *)(seen_col[k] + off);
+                                                if (COUNT)
+                                                    ++pairs;
+                                                call(&A.agent, &B.agent, ctx);
+                                            }
                                         }
                                     }
                                 }
-                            }
+                    }
                 }
             } else {
                 /* The group's cells in x, y, z order (z fastest), their
@@ -619,7 +693,7 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     for (int k = 0; k < 4; ++k)
         v.radii[k] = args.radii[k];
     const unsigned pairs = tay_see_tile<T, SEER_WORDS, SEEN_WORDS, NARROW, SKIP_SELF, SEEN_GRID, DIMS, COUNT,
-                                        CTX_WORDS>(v, tile * args.tile_seers, args.tile_seers, args.width,
+                                        CTX_WORDS, false>(v, tile * args.tile_seers, args.tile_seers, args.width,
                                                    args.lane_walk, call);
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);
@@ -663,7 +737,7 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
 #pragma unroll
         for (int d = 0; d < 4; ++d)
             v.radii[d] = tay_uniform(args.radii[4u * b + d]);
-        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS>(
+        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS, true>(
             v, tile * 64u, 64u, args.width, args.lane_walk, call);
         unsigned q = 0;
         if ((threadIdx.x & 63u) == 0u)

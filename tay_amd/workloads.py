@@ -323,6 +323,16 @@ def ensemble(instances=256, seed=7, counts=(1000, 64000), boxes=(200.0, 1500.0),
     return out
 
 
+def density_part(records, radius, target, min_factor, max_factor):
+    """Cell side for about `target` agents per cell over the bounding box of
+    the records' positions, clamped to [min_factor, max_factor] x radius: the
+    rule tay_gpu_batch_density_part_sizes applies on the device (3D)."""
+    p = np.stack([records["p"][:, k] for k in range(3)], 1).astype(np.float64)
+    vol = float(np.prod(np.maximum(p.max(0) - p.min(0), 0.0))) if len(p) else 0.0
+    side = (target * vol / len(p)) ** (1.0 / 3.0) if len(p) and vol > 0 else 0.0
+    return min(max(side, min_factor * radius), max_factor * radius)
+
+
 def cell_table_mb(side, part, headroom=2.0):
     """Whole megabytes of cell table for a cubic box of `side` at cell size
     `part`, with `headroom` per axis for the box to grow."""

@@ -121,6 +121,60 @@ def test_batch_autotune_restores_agents(gpu):
         assert_bitwise(x, y)
 
 
+def test_batch_density_part_sizes_and_autotune(gpu):
+    """Part sizes by agent density follow their rule (the host restatement
+    W.density_part, both clamps, equal bounds), and the density autotuner's
+    choice runs bit for bit like per-instance states with those sizes."""
+    import ctypes as C
+    ens = small_ensemble(6, seed=9)
+    radius = [m.passes[0].radius for _, m, _ in ens]
+    tables = [W.cell_table_mb(side, 0.5 * r) for (_, _, side), r in zip(ens, radius)]
+    fptr = C.POINTER(C.c_float)
+    print("case: density part sizes, 6 instances", flush=True)
+    with W.Batch(gpu, ens, radius, tables) as b:
+        for target, lo, hi in ((4.0, 0.5, 4.0), (1e-6, 0.5, 4.0), (1e9, 0.5, 4.0), (8.0, 1.25, 1.25)):
+            parts = np.zeros((len(ens), 4), np.float32)
+            assert gpu.tay_gpu_batch_density_part_sizes(b.batch, target, lo, hi, parts.ctypes.data_as(fptr)), \
+                gpu.tay_gpu_last_error().decode()
+            want = np.array([W.density_part(rec, r, target, lo, hi) for (rec, _, _), r in zip(ens, radius)])
+            np.testing.assert_allclose(parts[:, 0], want, rtol=1e-6)
+            assert (parts[:, 1] == parts[:, 0]).all() and (parts[:, 2] == parts[:, 0]).all()
+            if target < 1e-3 or lo == hi:
+                np.testing.assert_allclose(parts[:, 0], lo * np.array(radius), rtol=1e-6)
+            if target > 1e3:
+                np.testing.assert_allclose(parts[:, 0], hi * np.array(radius), rtol=1e-6)
+        assert not gpu.tay_gpu_batch_density_part_sizes(b.batch, 0.0, 0.5, 4.0, parts.ctypes.data_as(fptr))
+        assert b"agents per cell" in gpu.tay_gpu_last_error()
+        assert not gpu.tay_gpu_batch_density_part_sizes(b.batch, 4.0, 2.0, 1.0, parts.ctypes.data_as(fptr))
+        assert b"cell factor bounds" in gpu.tay_gpu_last_error()
+
+        print("case: density autotune 3 targets x 2 lane walks", flush=True)
+        before = b.agents()
+        targets = (C.c_float * 3)(1.0, 4.0, 16.0)
+        widths = (C.c_uint * 1)(64)
+        walks = (C.c_uint * 2)(16, 64)
+        table = (C.c_double * 6)()
+        bt, bw, bl = C.c_float(), C.c_uint(), C.c_uint()
+        chosen = np.zeros((len(ens), 4), np.float32)
+        assert gpu.tay_gpu_batch_autotune_density(b.batch, targets, 3, 0.5, 4.0, widths, 1, walks, 2, 2, table,
+                                                  C.byref(bt), chosen.ctypes.data_as(fptr), C.byref(bw),
+                                                  C.byref(bl)), gpu.tay_gpu_last_error().decode()
+        assert bt.value in (1.0, 4.0, 16.0) and bw.value == 64 and bl.value in (16, 64)
+        assert all(x > 0 for x in table)
+        applied = np.zeros((len(ens), 4), np.float32)
+        assert gpu.tay_gpu_batch_get_part_sizes(b.batch, applied.ctypes.data_as(fptr))
+        np.testing.assert_array_equal(applied, chosen)
+        want = np.array([W.density_part(rec, r, bt.value, 0.5, 4.0) for (rec, _, _), r in zip(ens, radius)])
+        np.testing.assert_allclose(chosen[:, 0], want, rtol=1e-6)
+        for x, y in zip(before, b.agents()):
+            assert_bitwise(x, y)
+        b.run(2, graph=True)
+        got = b.agents()
+    want, _ = per_instance(gpu, ens, [float(x) for x in chosen[:, 0]], tables, 2)
+    for x, y in zip(got, want):
+        assert_bitwise(x, y)
+
+
 def test_batch_of_more_than_256_instances(gpu):
     """Instances beyond 256 (the batch sort works per instance, whatever
     their number): still identical to per-instance states."""
@@ -147,12 +201,14 @@ def test_batch_rejects_bad_arguments(gpu):
 
 def test_full_size_ensemble_matches_per_instance_states_and_oracle(gpu, oracle, models):
     """The benchmark's ensemble at its real shape: 256 instances of 1k-64k
-    agents (3.9M in all), box sides 200-1500, radii 10-40, cell factor 1.5;
-    two graph-replayed steps, every instance bit for bit equal to its own
-    TAY_GPU_Z_GRID state, and four instances (the smallest, the largest and
-    two seeded others) equal to the oracle's."""
+    agents (3.9M in all), box sides 200-1500, radii 10-40, cells of about 8
+    agents by density (0.5 to 4 radii); two graph-replayed steps, every
+    instance bit for bit equal to its own TAY_GPU_Z_GRID state, and four
+    instances (the smallest, the largest and two seeded others) equal to the
+    oracle's."""
     ens = W.ensemble(256, seed=7)
-    part, tables = parts_and_tables(ens, 1.5)
+    part = [W.density_part(rec, m.passes[0].radius, 8.0, 0.5, 4.0) for rec, m, _ in ens]
+    tables = [W.cell_table_mb(side, p) for (_, _, side), p in zip(ens, part)]
     print("case: full-size ensemble, 256 instances, 2 steps, graph", flush=True)
     with W.Batch(gpu, ens, part, tables, count_pairs=True) as b:
         b.run(2, graph=True)
