@@ -96,7 +96,7 @@ SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of 
 # accumulators (8), the seen side reads p, v, density and pressure (8);
 # density, the seer reads p and density (4) and writes density (1), the seen
 # side reads p (3).
-SPH_TRAFFIC = {"particles": 2_000_000, "kernel": "see kernels of commit c7e55b7",
+SPH_TRAFFIC = {"particles": 2_000_000, "kernel": "see kernels of commit c7e55b7", "order": "reference",
                "forces": int((69859.7 * 2.000 + 46875.0 * 1.000) * 1024),
                "density": int((19950.6 * 2.000 + 7812.5 * 1.000) * 1024)}
 B_SPH_FORCES_PER_PARTICLE = 4 * (15 + 8 + 8)
@@ -549,31 +549,42 @@ def bench_ensemble(lib, args):
     return out
 
 
+SEE_ORDERS = {"reference": 0, "balanced": 1}   # TAY_SEE_ORDER_REFERENCE / _BALANCED
+
+
 def bench_sph(lib, args):
     """SPH dam break: two see passes (density, forces) and two act passes
-    per step on 2M particles."""
+    per step on 2M particles, in the see order args.sph_see_order (the
+    timed line) and, for comparison, in the other order."""
     from tay_amd import api
     from tay_amd import workloads as W
     recs, lo, hi = W.sph_particles(args.sph_particles, h=1.0)
     ctx = W.sph_context(h=1.0, dt=1e-3, box_min=lo, box_max=hi)
     model = W.sph_model(ctx, 1.0)
     steps, warm = args.sph_steps, 2
+    order = SEE_ORDERS[args.sph_see_order]
     with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as twin:
         lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
+        lib.tay_set_see_order(twin.state, order)
         lib.tay_set_pair_counting(twin.state, 1)
         twin.run(warm)
         twin.run(steps)
         pairs = [lib.pair_counts(twin.state, i)[1] for i in (0, 2)]
-    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
-        lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
-        sim.run(warm)
-        lib.tay_fetch_agents(sim.state, None)
-        first = sim.agents()
-        wall, times, fetch = timed_run(lib, sim, steps)
-        last = sim.agents()
-        see = [times[i][0] for i in (0, 2)]
-        act = [times[i][0] for i in (1, 3)]
-    progress("sph: timed run done")
+    runs = {}
+    for name in [args.sph_see_order] + [n for n in SEE_ORDERS if n != args.sph_see_order]:
+        with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
+            lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
+            lib.tay_set_see_order(sim.state, SEE_ORDERS[name])
+            sim.run(warm)
+            lib.tay_fetch_agents(sim.state, None)
+            first = sim.agents()
+            runs[name] = timed_run(lib, sim, steps) + (first, sim.agents())
+        progress("sph: timed run done (%s see order)" % name)
+    wall, times, fetch, first, last = runs[args.sph_see_order]
+    see = [times[i][0] for i in (0, 2)]
+    act = [times[i][0] for i in (1, 3)]
+    other = [n for n in runs if n != args.sph_see_order][0]
+    o_wall, o_times, _, _, _ = runs[other]
     inside = []
     for a in (first, last):
         box, ball = pair_counts_at(lib, a["p"], 1.0, 1.0)
@@ -582,11 +593,11 @@ def bench_sph(lib, args):
         inside.append(((ball + len(a)) / (box + len(a)), ball / box))
     share_d = 0.5 * (inside[0][0] + inside[1][0])
     share_f = 0.5 * (inside[0][1] + inside[1][1])
-    measured = len(recs) == SPH_TRAFFIC["particles"]
+    measured = len(recs) == SPH_TRAFFIC["particles"] and args.sph_see_order == SPH_TRAFFIC["order"]
     f_d = f_pair_of(SPH_DENSITY_F, {"inside": share_d})
     f_f = f_pair_of(SPH_FORCES_F, {"inside": share_f})
     out = {"workload": "SPH dam break, %d particles" % len(recs), "particles": len(recs), "h": 1.0, "dt": 1e-3,
-           "steps": steps, "p_see_per_step": [p / steps for p in pairs],
+           "steps": steps, "see_order": args.sph_see_order, "p_see_per_step": [p / steps for p in pairs],
            "pairs_per_s": sum(pairs) / (sum(see) / 1e3), "steps_per_s": steps / wall,
            "steps_per_s_resident": steps / (wall - fetch), "ms_per_step": wall * 1e3 / steps,
            "see_ms_per_step": [t / steps for t in see], "act_ms_per_step": [t / steps for t in act],
@@ -601,7 +612,10 @@ def bench_sph(lib, args):
                                                  "see (tay_pass_0, sph_density)", f_d, "",
                                                  traffic=SPH_TRAFFIC["density"] if measured else None,
                                                  traffic_bytes_algorithmic=B_SPH_DENSITY_PER_PARTICLE * len(recs),
-                                                 branch_shares={"inside": share_d}))}
+                                                 branch_shares={"inside": share_d})),
+           other + "_see_order": {"steps_per_s": steps / o_wall, "ms_per_step": o_wall * 1e3 / steps,
+                                  "see_ms_per_step": [o_times[i][0] / steps for i in (0, 2)],
+                                  "pairs_per_s": sum(pairs) / (sum(o_times[i][0] for i in (0, 2)) / 1e3)}}
     if not args.no_cpu_baseline:
         base = {}
         for threads in (args.cpu_threads, 1):
@@ -740,6 +754,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 10k plain-agent measurement")
     ap.add_argument("--sph-particles", type=int, default=2_000_000)
     ap.add_argument("--sph-steps", type=int, default=500)
+    ap.add_argument("--sph-see-order", choices=sorted(SEE_ORDERS), default="balanced",
+                    help="see order of the timed SPH run (the other order is timed too, for comparison)")
     ap.add_argument("--no-sph", action="store_true", help="skip the SPH measurement")
     ap.add_argument("--adjoint-agents", type=int, default=65536)
     ap.add_argument("--adjoint-iters", type=int, default=200)

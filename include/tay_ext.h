@@ -42,16 +42,37 @@ int tay_get_pass_time(TayState *state, unsigned pass_index, double *ms, unsigned
 int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
 
 /* Seers per lane group when a see tile starts (64, 32, 16 or 8; default
- * 64), as tay_gpu_program_set_tile_width; results do not depend on it. */
+ * 64), as tay_gpu_program_set_tile_width.  In the reference see order
+ * results do not depend on it; in the balanced order it is part of the
+ * order in which each seer meets its pairs. */
 int tay_set_tile_width(TayState *state, unsigned width);
 
 /* Lane-walk width of the see kernels (0 to 64), as
- * tay_gpu_program_set_lane_walk; results do not depend on it. */
+ * tay_gpu_program_set_lane_walk; as the tile width, part of the balanced
+ * order only. */
 int tay_set_lane_walk(TayState *state, unsigned width);
 
 /* Seers per see tile (0 = by the seer count, the default; 64, 32 or 16),
- * as tay_gpu_program_set_tile_seers; results do not depend on it. */
+ * as tay_gpu_program_set_tile_seers; as the tile width, part of the
+ * balanced order only. */
 int tay_set_tile_seers(TayState *state, unsigned seers);
+
+/* Order in which a seer meets its pairs in see passes over a GPU Z-grid.
+ * TAY_SEE_ORDER_REFERENCE (the default) is the reference CPU grids' sweep
+ * (cells x, then y, then z, each in storage order; tay/cpu_grid.c:222-261),
+ * so float sums are bit-identical to a CPU grid's with the same storage
+ * order.  TAY_SEE_ORDER_BALANCED takes the candidates of a tile's cells in
+ * strided chunks, each an even sample of the tile's neighbourhood, which
+ * keeps more lanes busy in the see kernel: the same pairs and neighbour
+ * counts, float sums in another fixed order, set by the seer count, the
+ * tile seers, the tile width and the lane walk.  Takes effect at the next
+ * tay_run (the program is compiled again).  Returns 0 for an unknown
+ * order.  No reference counterpart. */
+typedef enum TaySeeOrder {
+    TAY_SEE_ORDER_REFERENCE = 0,
+    TAY_SEE_ORDER_BALANCED = 1,
+} TaySeeOrder;
+int tay_set_see_order(TayState *state, int order);
 
 /* The device space of a group, valid while the simulation runs (NULL before
  * the first tay_run and for groups not on the GPU). */

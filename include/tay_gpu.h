@@ -153,6 +153,8 @@ typedef struct TayGpuPassDesc {
     float radii[4];             /* see passes only */
     int self_see;               /* see passes only; same-space pairs (i, i) skipped when 0 */
     unsigned context_size;      /* bytes copied by tay_gpu_program_set_context */
+    int see_order;              /* see passes over a Z-grid: 0 the reference sweep order, 1 the balanced
+                                   order (TaySeeOrder, tay_ext.h) */
 } TayGpuPassDesc;
 
 /* Composes the model sources and one generated kernel per pass into a single
@@ -178,6 +180,7 @@ typedef struct TayGpuPassShape {
     int dims;
     int skip_self;              /* 1 when seer and seen are one group and self_see is 0 */
     unsigned context_size;      /* bytes of the pass context */
+    int see_order;              /* as TayGpuPassDesc::see_order */
 } TayGpuPassShape;
 
 /* Composes and compiles a program to a gfx950 code object without loading

@@ -137,6 +137,9 @@ typedef struct TayPass {
     PairingFunc pairing;
     PairCounts counts;      /* summed over the steps of the last run */
     unsigned seer_limit;    /* see passes: only the first seer_limit seers (0: all) */
+    /* GPU Z-grid seen spaces: see order and tile layout (the state's) */
+    int see_order;
+    unsigned tile_seers, tile_width, lane_walk;
 } TayPass;
 
 typedef struct TayState {
@@ -151,6 +154,10 @@ typedef struct TayState {
     int recompile;
     char *sources[MAX_SOURCES];
     unsigned sources_count;
+    int see_order;          /* 0: reference sweep order, 1: balanced */
+    unsigned tile_seers;    /* 0: by the seer count */
+    unsigned tile_width;
+    unsigned lane_walk;
 } TayState;
 
 /* ------------------------------------------------------------ thread pool */
@@ -280,7 +287,12 @@ static void set_error(TayState *state, TayError error, const char *message) {
 }
 
 TayState *tay_create_state() {
-    return (TayState *)calloc(1, sizeof(TayState));
+    TayState *state = (TayState *)calloc(1, sizeof(TayState));
+    if (state) {
+        state->tile_width = 64;     /* the GPU library's defaults */
+        state->lane_walk = 64;
+    }
+    return state;
 }
 
 void tay_destroy_state(TayState *state) {
@@ -936,6 +948,43 @@ int tay_set_order_bits(TayState *state, TayGroup *group, int bits) {
     return 1;
 }
 
+/* See order and tile layout of the GPU Z-grid statement (oracle.h). */
+int tay_set_see_order(TayState *state, int order) {
+    if (order != 0 && order != 1) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see order must be TAY_SEE_ORDER_REFERENCE or TAY_SEE_ORDER_BALANCED");
+        return 0;
+    }
+    state->see_order = order;
+    return 1;
+}
+
+int tay_set_tile_seers(TayState *state, unsigned seers) {
+    if (seers != 0 && seers != 64 && seers != 32 && seers != 16) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "seers per tile must be 0, 64, 32 or 16");
+        return 0;
+    }
+    state->tile_seers = seers;
+    return 1;
+}
+
+int tay_set_tile_width(TayState *state, unsigned width) {
+    if (width != 64 && width != 32 && width != 16 && width != 8) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "tile width must be 64, 32, 16 or 8");
+        return 0;
+    }
+    state->tile_width = width;
+    return 1;
+}
+
+int tay_set_lane_walk(TayState *state, unsigned width) {
+    if (width > 64) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "lane walk width must be at most 64");
+        return 0;
+    }
+    state->lane_walk = width;
+    return 1;
+}
+
 /* The geometry is the reference OpenCL Z-grid's (tay/ocl_grid.c:128-136):
  * count_i = ceil(size_i / part_i), cell_size_i = size_i / count_i with
  * size_i = max_i - min_i + 0.001 part_i as in every reference grid.  That
@@ -1122,6 +1171,137 @@ static void gpu_z_grid_see_seen_one(TayPass *p, Slice seer, PairCounts *c) {
     }
 }
 
+/* Balanced see order: the GPU library's see kernel (jit_device.h,
+ * tay_see_tile) takes the seers of a seer group in tiles of S consecutive
+ * storage slots and each tile in lane groups: a group starts `width` lanes
+ * wide (the tile width, at most S); when its box grown by the radii spans
+ * more than `limit` cells it halves while wider than the lane walk, else
+ * every member sweeps its own cells in the reference order (the per-lane
+ * walk); otherwise the group sweeps the cells of its box x, then y, then z,
+ * 64 cells at a time.  The M candidates of such a segment, in sweep order,
+ * are cut into P = 64 / BALANCE_RUN parts of L = ceil(M / P); chunk k holds
+ * run k (BALANCE_RUN consecutive positions) of every part, slot
+ * j * BALANCE_RUN + t holding position j * L + k * BALANCE_RUN + t when that
+ * lies inside part j and the segment, and every member meets its passing
+ * candidates chunk by chunk, slot by slot.  After a group the width doubles
+ * back while the next start is aligned to it.  Float expressions follow the
+ * kernel's. */
+#define BALANCE_RUN 4u      /* the GPU library's TAY_BALANCE_RUN */
+
+static Slice seer_slice_of(TayPass *p, unsigned beg, unsigned end);
+static Slice seen_slice(TayPass *p, unsigned beg, unsigned end);
+
+/* seers per tile chosen by the seer count: 64 while that gives each of a
+ * device's SIMDs two tiles, else 32, else 16 (jit.cpp, auto_tile_seers) */
+static unsigned auto_tile_seers(unsigned seers) {
+    const unsigned simds = 1024u;   /* 256 compute units of 4 SIMDs */
+    for (unsigned s = 64u; s > 16u; s >>= 1)
+        if ((seers + s - 1u) / s >= 2u * simds)
+            return s;
+    return 16u;
+}
+
+static int gpu_cell_axis(float p, float origin, float size, int count) {
+    int c = (int)floorf((p - origin) / size);
+    c = c < 0 ? 0 : c;
+    return c > count - 1 ? count - 1 : c;
+}
+
+static void gpu_z_grid_balanced_sweep(TayPass *p, unsigned m0, unsigned m1, const int *lo, const int *hi,
+                                      PairCounts *c) {
+    const Space *s = &p->seen_group->space;
+    Slice seers = seer_slice_of(p, 0, 0);
+    const unsigned nz = (unsigned)(hi[2] - lo[2] + 1), nyz = nz * (unsigned)(hi[1] - lo[1] + 1);
+    const unsigned swept = nyz * (unsigned)(hi[0] - lo[0] + 1);
+    for (unsigned c0 = 0; c0 < swept; c0 += 64u) {
+        const unsigned n = swept - c0 < 64u ? swept - c0 : 64u;
+        unsigned cbeg[64], pre[64], total = 0;
+        for (unsigned q = 0; q < n; ++q) {
+            const unsigned cell = c0 + q, rem = cell % nyz;
+            Int4 idx = {{lo[0] + (int)(cell / nyz), lo[1] + (int)(rem / nz), lo[2] + (int)(rem % nz), 0}};
+            const CellRange *r = s->ranges + row_major(idx, s->cell_counts, s->dims);
+            cbeg[q] = r->beg;
+            pre[q] = total;
+            total += r->end - r->beg;
+        }
+        if (!total)
+            continue;
+        const unsigned parts = 64u / BALANCE_RUN;
+        const unsigned part_len = (total + parts - 1u) / parts;
+        const unsigned chunks = (part_len + BALANCE_RUN - 1u) / BALANCE_RUN;
+        for (unsigned k = 0; k < chunks; ++k)
+            for (unsigned slot = 0; slot < 64u; ++slot) {
+                const unsigned o = k * BALANCE_RUN + slot % BALANCE_RUN;
+                const unsigned pos = slot / BALANCE_RUN * part_len + o;
+                if (o >= part_len || pos >= total)
+                    continue;
+                unsigned q = 0;     /* the last cell whose prefix is at most pos */
+                while (q + 1u < n && pre[q + 1u] <= pos)
+                    ++q;
+                const unsigned j = cbeg[q] + (pos - pre[q]);
+                for (unsigned i = m0; i < m1; ++i) {
+                    Slice one = seers;
+                    one.beg = i;
+                    one.end = i + 1;
+                    p->pairing(p, one, seen_slice(p, j, j + 1), c);
+                }
+            }
+    }
+}
+
+/* Seers [beg, end) of the seer group, whole tiles of S, out of `seers`. */
+static void gpu_z_grid_see_tiles(TayPass *p, unsigned beg, unsigned end, unsigned seers, unsigned S,
+                                 PairCounts *c) {
+    const Space *s = &p->seen_group->space;
+    const TayGroup *sg = p->seer_group;
+    const int dims = s->dims;
+    const float *r = p->radii.arr;
+    float ext[3] = {0.0f, 0.0f, 0.0f};
+    unsigned limit = 1;
+    for (int k = 0; k < dims; ++k) {
+        if (!p->seen_group->is_point)
+            ext[k] = s->gz_max_extent[k];
+        limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / s->cell_sizes.arr[k]) + 3u;
+    }
+    const unsigned width0 = p->tile_width < S ? p->tile_width : S;
+    for (unsigned i0 = beg; i0 < end; i0 += S) {
+        const unsigned n_active = seers - i0 < S ? seers - i0 : S;
+        unsigned gbeg = 0, width = width0;
+        while (gbeg < n_active) {
+            const unsigned m0 = i0 + gbeg, m1 = i0 + (gbeg + width < n_active ? gbeg + width : n_active);
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            unsigned cells = 1;
+            for (int k = 0; k < dims; ++k) {
+                float tmin = INFINITY, tmax = -INFINITY;
+                for (unsigned i = m0; i < m1; ++i) {
+                    const char *a = sg->storage + (size_t)sg->agent_size * i;
+                    tmin = fminf(tmin, POS(a)[k]);
+                    tmax = fmaxf(tmax, sg->is_point ? POS(a)[k] : MAXC(a)[k]);
+                }
+                const float mlo = (fabsf(tmin) + r[k] + ext[k]) * 0x1p-20f;
+                const float mhi = (fabsf(tmax) + r[k]) * 0x1p-20f;
+                lo[k] = gpu_cell_axis(((tmin - r[k]) - ext[k]) - mlo, s->origin.arr[k], s->cell_sizes.arr[k],
+                                      s->cell_counts.arr[k]);
+                hi[k] = gpu_cell_axis((tmax + r[k]) + mhi, s->origin.arr[k], s->cell_sizes.arr[k],
+                                      s->cell_counts.arr[k]);
+                cells *= (unsigned)(hi[k] - lo[k] + 1);
+            }
+            if (cells > limit && width > 1u && width > p->lane_walk) {
+                width >>= 1;
+                continue;
+            }
+            if (cells > limit && width > 1u) {
+                gpu_z_grid_see_seen_one(p, seer_slice_of(p, m0, m1), c);
+            } else if (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2]) {
+                gpu_z_grid_balanced_sweep(p, m0, m1, lo, hi, c);
+            }
+            gbeg += width;
+            while (width < width0 && (gbeg & (2u * width - 1u)) == 0u)
+                width <<= 1;
+        }
+    }
+}
+
 /* -------------------------------------------------------------- see/act */
 
 typedef struct {
@@ -1154,6 +1334,15 @@ static void see_task(void *task_p, int thread_i, PairCounts *c) {
     int dims = ss->dims < p->seen_group->space.dims ? ss->dims : p->seen_group->space.dims;
     unsigned beg, end;
     unsigned seers = p->seer_limit && p->seer_limit < ss->count ? p->seer_limit : ss->count;
+    if (p->see_order == 1 && p->seen_group->space.type == TAY_GPU_Z_GRID && ss->type != TAY_CPU_GRID &&
+        ss->type != TAY_CPU_Z_GRID) {
+        /* balanced order: each thread takes whole tiles */
+        const unsigned S = p->tile_seers ? p->tile_seers : auto_tile_seers(ss->count);
+        unsigned tb, te;
+        thread_range((seers + S - 1u) / S, thread_i, &tb, &te);
+        gpu_z_grid_see_tiles(p, tb * S, te * S < seers ? te * S : seers, seers, S, c);
+        return;
+    }
     thread_range(seers, thread_i, &beg, &end);
 
     if (ss->type != TAY_CPU_GRID && ss->type != TAY_CPU_Z_GRID) {
@@ -1338,6 +1527,10 @@ int tay_run(TayState *state, int steps) {
             TayPass *p = state->passes + pi;
             PassTask task = {p};
             if (p->type == PASS_SEE) {
+                p->see_order = state->see_order;
+                p->tile_seers = state->tile_seers;
+                p->tile_width = state->tile_width;
+                p->lane_walk = state->lane_walk;
                 double t0 = now_seconds();
                 run_tasks(see_task, &task, &p->counts);
                 state->see_seconds += now_seconds() - t0;

@@ -46,6 +46,17 @@ void oracle_morton_decode(unsigned key, int dims, int *x, int *y, int *z);
  * order inside a cell; the GPU library exports the same call. */
 int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
 
+/* See order of the GPU Z-grid statement (0: the reference sweep order, the
+ * default; 1: the balanced order of the GPU library's see kernel, whose
+ * tile layout the next three calls set as the GPU library's calls of the
+ * same names do: seers per tile (0: by the seer count, for a device of 1024
+ * SIMDs), starting lane-group width, lane-walk width).  The GPU library
+ * exports the same calls (include/tay_ext.h). */
+int tay_set_see_order(TayState *state, int order);
+int tay_set_tile_seers(TayState *state, unsigned seers);
+int tay_set_tile_width(TayState *state, unsigned width);
+int tay_set_lane_walk(TayState *state, unsigned width);
+
 /* Wall-clock seconds spent in see passes during the last tay_run. */
 double oracle_get_see_seconds(TayState *state);
 

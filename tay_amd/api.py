@@ -37,6 +37,8 @@ TAY_ERROR_CAPACITY = 7
 # TayFetchMode (include/tay_ext.h)
 TAY_FETCH_EVERY_RUN = 0
 TAY_FETCH_ON_DEMAND = 1
+TAY_SEE_ORDER_REFERENCE = 0
+TAY_SEE_ORDER_BALANCED = 1
 
 # TayGpuSpaceKind / TayGpuPassType (include/tay_gpu.h)
 TAY_GPU_KIND_Z_GRID = 1
@@ -59,7 +61,8 @@ class TayGpuPassShape(C.Structure):
     _fields_ = [("type", C.c_int), ("func_name", C.c_char_p),
                 ("seer_agent_size", C.c_uint), ("seen_agent_size", C.c_uint),
                 ("seer_is_point", C.c_int), ("seen_is_point", C.c_int),
-                ("seen_kind", C.c_int), ("dims", C.c_int), ("skip_self", C.c_int), ("context_size", C.c_uint)]
+                ("seen_kind", C.c_int), ("dims", C.c_int), ("skip_self", C.c_int), ("context_size", C.c_uint),
+                ("see_order", C.c_int)]
 
 
 class TayGpuBatchPass(C.Structure):
@@ -112,6 +115,7 @@ _GPU = {
     "tay_set_tile_width": (_i, [_vp, _u]),
     "tay_set_lane_walk": (_i, [_vp, _u]),
     "tay_set_tile_seers": (_i, [_vp, _u]),
+    "tay_set_see_order": (_i, [_vp, _i]),
     "tay_set_order_bits": (_i, [_vp, _vp, _i]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),

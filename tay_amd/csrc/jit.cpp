@@ -130,6 +130,7 @@ TayGpuPassShape shape_of(const TayGpuPassDesc &d) {
         sh.seen_is_point = d.seen->is_point;
         sh.seen_kind = d.seen->kind;
         sh.skip_self = d.seer == d.seen && !d.self_see;
+        sh.see_order = d.see_order;
     }
     return sh;
 }
@@ -217,13 +218,13 @@ std::string compose(const char *const *sources, unsigned sources_count, const Ta
             append(text,
                    " void tay_pass_%u(TaySeeArgs args) {\n"
                    "    typedef tay_see_traits<decltype(%s)> T;\n"
-                   "    tay_see_tiles<T, %uu, %uu, %d, %s, %s, %d, %s, %uu>(args,\n"
+                   "    tay_see_tiles<T, %uu, %uu, %d, %s, %s, %d, %s, %uu, %s>(args,\n"
                    "        [](T::seer *a, T::seen *b, T::context *c) { %s(a, b, c); });\n"
                    "}\n",
                    i, p.func_name, p.seer_agent_size / 4u, p.seen_agent_size / 4u,
                    narrow_kind(p.seer_is_point, p.seen_is_point), p.skip_self ? "true" : "false",
                    p.seen_kind == TAY_GPU_KIND_Z_GRID ? "true" : "false", p.dims, count_pairs ? "true" : "false",
-                   (p.context_size + 3u) / 4u,
+                   (p.context_size + 3u) / 4u, p.see_order ? "true" : "false",
                    p.func_name);
         } else {
             append(text,

@@ -72,6 +72,14 @@ template <class C, unsigned WORDS> struct tay_ctx_holder<C, WORDS, true> {
 
 #define TAY_SEE_BLOCK 256u
 
+/* Candidates per run of a chunk in the balanced see order (a power of two,
+ * 1 to 64).  Measured on the 2M-particle SPH dam break (force pass, first
+ * steps): 1.17 ms per pass in the sweep order; runs of 1, 4, 16, 32:
+ * 0.99, 0.98, 1.17, 1.12 ms. */
+#ifndef TAY_BALANCE_RUN
+#define TAY_BALANCE_RUN 4u
+#endif
+
 __device__ __forceinline__ float tay_wave_min(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1)
@@ -104,6 +112,49 @@ __device__ __forceinline__ unsigned tay_wave_sum(unsigned v) {
     for (int o = 32; o > 0; o >>= 1)
         v += __shfl_xor(v, o);
     return v;
+}
+
+/* Inclusive prefix sum over the wavefront's lanes. */
+__device__ __forceinline__ unsigned tay_wave_incl_sum(unsigned v) {
+    const unsigned lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(v, o);
+        v += lane >= (unsigned)o ? t : 0u;
+    }
+    return v;
+}
+
+/* Cell indices of an agent's tag: part_i holds the Morton key of the cell
+ * the sort stored it in (3D 10 bits per axis interleaved x, y, z from bit
+ * 0; 2D 16 bits, x, y; 1D the index itself). */
+__device__ __forceinline__ unsigned tay_compact3(unsigned v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030C30C3u;
+    v = (v | (v >> 4)) & 0x0300F00Fu;
+    v = (v | (v >> 8)) & 0xFF0000FFu;
+    return (v | (v >> 16)) & 0x000003FFu;
+}
+__device__ __forceinline__ unsigned tay_compact2(unsigned v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    return (v | (v >> 8)) & 0x0000FFFFu;
+}
+__device__ __forceinline__ void tay_cell_of_tag(unsigned part_i, int dims, int *c) {
+    if (dims == 3) {
+        c[0] = (int)tay_compact3(part_i);
+        c[1] = (int)tay_compact3(part_i >> 1);
+        c[2] = (int)tay_compact3(part_i >> 2);
+    } else if (dims == 2) {
+        c[0] = (int)tay_compact2(part_i);
+        c[1] = (int)tay_compact2(part_i >> 1);
+        c[2] = 0;
+    } else {
+        c[0] = (int)part_i;
+        c[1] = c[2] = 0;
+    }
 }
 
 /* Logical block order that hands each XCD one contiguous run of tiles: the
@@ -219,9 +270,23 @@ struct TaySeeView {
  * to `seers`) is the size of the lane groups a tile starts with; a group of at most lane_walk
  * lanes whose box is still too wide walks per lane instead of splitting
  * further (0: always split).  Returns the pairs this lane passed to the
- * model when COUNT is set. */
+ * model when COUNT is set.
+ *
+ * BALANCED (the balanced see order): a swept group takes its cells 64 at a
+ * time; the M candidates of those cells, in sweep order, are cut into
+ * P = 64 / TAY_BALANCE_RUN parts of L = ceil(M / P), and chunk k holds run k
+ * (TAY_BALANCE_RUN consecutive candidates) of every part: slot
+ * j * TAY_BALANCE_RUN + t holds sweep position j * L + k * TAY_BALANCE_RUN + t,
+ * slots past the end of their part left empty.  A run of consecutive
+ * candidates lies on one side of the group, where some seers pass most of
+ * them and others none; a chunk made of runs from every part of the
+ * neighbourhood spreads each seer's pairs over the chunks more evenly, so
+ * fewer compact rounds carry the same pairs, while runs keep the loads of
+ * a round on few cache lines.  Each seer meets the same pairs in that
+ * fixed order instead of the sweep order; TAY_BALANCE_RUN 64 (or M <= 64)
+ * is the sweep order. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
-          bool COUNT, unsigned CTX_WORDS, bool ROW_WALK, class Call>
+          bool COUNT, unsigned CTX_WORDS, bool ROW_WALK, bool BALANCED, class Call>
 __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned seers, unsigned width0,
                                                 unsigned lane_walk,
                                                 Call call) {
@@ -234,6 +299,7 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     __shared__ __align__(16) float chunk_lo[WAVES][64][4];
     __shared__ float chunk_hi[SEEN_BOX ? WAVES : 1u][64][4];
     __shared__ __align__(8) unsigned cell_ranges[SEEN_GRID ? WAVES : 1u][64][2];
+    __shared__ unsigned lane_cell[BALANCED ? WAVES : 1u][64];   /* balanced order: each lane's cell in the segment */
 
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
@@ -269,6 +335,7 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     unsigned fill = 0;      /* slots used in the current chunk (uniform) */
     unsigned cand = 0;      /* lane k: storage index of the chunk's slot k */
     unsigned self_slot = 64u; /* the chunk slot holding this lane's own seer, 64: none */
+    unsigned long long chunk_valid = ~0ull;  /* balanced order: the chunk's occupied slots */
 
     /* Point pairs with one radius on every axis: !(|d_k| > r) for all k is
      * !(max_k |d_k| > r) (the maximum ignores a NaN difference exactly as
@@ -330,6 +397,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         if (SKIP_SELF && self_slot < 64u)
             mask &= ~(1ull << self_slot);
         self_slot = 64u;
+        if (BALANCED)
+            mask &= chunk_valid;
         mask = member ? mask : 0ull;
         if (COUNT)
             pairs += (unsigned)__popcll(mask);
@@ -610,6 +679,86 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                                 }
                     }
                 }
+            } else if (BALANCED) {
+                /* The group's cells in x, y, z order (z fastest), 64 at a
+                 * time, one cell per lane: the cell's first slot and the
+                 * number of candidates before it (exclusive prefix sum) go
+                 * to the wavefront's LDS slots; slot s of the segment's
+                 * candidate sequence is then found by binary search. */
+                const bool empty = hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2];
+                const unsigned nz = (unsigned)(hi[2] - lo[2] + 1), nyz = nz * (unsigned)(hi[1] - lo[1] + 1);
+                const unsigned swept = empty ? 0u : nyz * (unsigned)(hi[0] - lo[0] + 1);
+                /* the seer's own slot, excluded from its masks: its cell
+                 * (the tag's Morton key, where the sort stored it) as an
+                 * index into the group's cells, 0xFFFFFFFF outside them */
+                unsigned own_cell = 0xFFFFFFFFu;
+                if (SKIP_SELF && member && !empty) {
+                    int oc[3];
+                    tay_cell_of_tag(A.w[0], DIMS, oc);
+                    if (oc[0] >= lo[0] && oc[0] <= hi[0] && oc[1] >= lo[1] && oc[1] <= hi[1] && oc[2] >= lo[2] &&
+                        oc[2] <= hi[2])
+                        own_cell = (unsigned)(oc[0] - lo[0]) * nyz + (unsigned)(oc[1] - lo[1]) * nz +
+                                   (unsigned)(oc[2] - lo[2]);
+                }
+                for (unsigned c0 = 0; c0 < swept; c0 += 64u) {
+                    const unsigned c = c0 + lane;
+                    unsigned cbeg = 0, cnt = 0;
+                    if (c < swept) {
+                        const unsigned x = (unsigned)lo[0] + c / nyz, rem = c % nyz;
+                        const unsigned y = (unsigned)lo[1] + rem / nz, z = (unsigned)lo[2] + rem % nz;
+                        const TayCellRange cr = v.ranges[x + y * cx + z * cxy];
+                        cbeg = cr.beg;
+                        cnt = cr.end - cr.beg;
+                    }
+                    const unsigned incl = tay_wave_incl_sum(cnt);
+                    const unsigned total = __builtin_amdgcn_readlane(incl, 63);
+                    /* lanes past the segment's cells hold prefix `total`,
+                     * which no slot reaches */
+                    cell_ranges[wave][lane][0] = cbeg;
+                    cell_ranges[wave][lane][1] = incl - cnt;
+                    tay_wave_sync();
+                    constexpr unsigned RUN = TAY_BALANCE_RUN, PARTS = 64u / TAY_BALANCE_RUN;
+                    static_assert(RUN >= 1u && RUN <= 64u && (RUN & (RUN - 1u)) == 0u, "TAY_BALANCE_RUN");
+                    const unsigned part_len = (total + PARTS - 1u) / PARTS;
+                    const unsigned chunks = (part_len + RUN - 1u) / RUN;
+                    /* the own seer's chunk and slot, as chunk * 64 + slot */
+                    unsigned own = 0xFFFFFFFFu;
+                    if (own_cell - c0 < 64u) {
+                        const unsigned q = own_cell - c0;
+                        const unsigned s_own = cell_ranges[wave][q][1] + (i - cell_ranges[wave][q][0]);
+                        const unsigned o = s_own % part_len;
+                        own = (o / RUN) * 64u + (s_own / part_len) * RUN + o % RUN;
+                    }
+                    {
+                        /* each lane's first slot: the last cell whose prefix
+                         * is at most its position (binary search); later
+                         * chunks move the lane on through its part */
+                        const unsigned s = (lane / RUN) * part_len + lane % RUN;
+                        const unsigned s0 = s < total ? s : total - 1u;
+                        unsigned q = 0;
+#pragma unroll
+                        for (unsigned step = 32; step; step >>= 1)
+                            q = cell_ranges[wave][q + step][1] <= s0 ? q + step : q;
+                        lane_cell[wave][lane] = q;
+                    }
+                    for (unsigned k = 0; k < chunks; ++k) {
+                        const unsigned o = k * RUN + lane % RUN;
+                        unsigned s = (lane / RUN) * part_len + o;
+                        const bool valid = o < part_len && s < total;
+                        chunk_valid = __ballot(valid);
+                        fill = 64u - (unsigned)__clzll(chunk_valid);
+                        if (valid) {
+                            unsigned q = lane_cell[wave][lane];
+                            while (q < 63u && cell_ranges[wave][q + 1u][1] <= s)
+                                ++q;
+                            lane_cell[wave][lane] = q;
+                            cand = cell_ranges[wave][q][0] + (s - cell_ranges[wave][q][1]);
+                        }
+                        if (SKIP_SELF && own >> 6 == k)
+                            self_slot = own & 63u;
+                        process();
+                    }
+                }
             } else {
                 /* The group's cells in x, y, z order (z fastest), their
                  * ranges loaded 64 at a time, one cell per lane, into the
@@ -671,7 +820,7 @@ __device__ __forceinline__ void tay_add_pairs(unsigned long long *counter, unsig
 
 /* Plain see kernel: wavefront k of the grid owns tile k. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
-          bool COUNT, unsigned CTX_WORDS, class Call>
+          bool COUNT, unsigned CTX_WORDS, bool BALANCED, class Call>
 __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
     constexpr unsigned WAVES = TAY_SEE_BLOCK / 64u;
     const unsigned tile = tay_xcd_block() * WAVES + (threadIdx.x >> 6);
@@ -693,7 +842,8 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     for (int k = 0; k < 4; ++k)
         v.radii[k] = args.radii[k];
     const unsigned pairs = tay_see_tile<T, SEER_WORDS, SEEN_WORDS, NARROW, SKIP_SELF, SEEN_GRID, DIMS, COUNT,
-                                        CTX_WORDS, false>(v, tile * args.tile_seers, args.tile_seers, args.width,
+                                        CTX_WORDS, false, BALANCED>(v, tile * args.tile_seers, args.tile_seers,
+                                                                  args.width,
                                                    args.lane_walk, call);
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);
@@ -737,7 +887,7 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
 #pragma unroll
         for (int d = 0; d < 4; ++d)
             v.radii[d] = tay_uniform(args.radii[4u * b + d]);
-        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS, true>(
+        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS, true, false>(
             v, tile * 64u, 64u, args.width, args.lane_walk, call);
         unsigned q = 0;
         if ((threadIdx.x & 63u) == 0u)

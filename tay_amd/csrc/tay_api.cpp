@@ -82,6 +82,7 @@ struct TayState {
     unsigned tile_width;
     unsigned lane_walk;
     unsigned tile_seers;        /* 0: chosen by the seer count */
+    int see_order;              /* TaySeeOrder of every see pass */
     int timing;
     int fetch_mode;
 };
@@ -262,6 +263,7 @@ int state_compile(TayState *state) {
         desc[i].radii[3] = p.radii.w;
         desc[i].self_see = p.self_see;
         desc[i].context_size = p.context_size;
+        desc[i].see_order = state->see_order;
     }
     static thread_local char log[1 << 16];
     log[0] = '\0';
@@ -302,6 +304,7 @@ TAY_API TayState *tay_create_state() {
     state->count_pairs = 0;
     state->tile_width = 64;
     state->tile_seers = 0;
+    state->see_order = TAY_SEE_ORDER_REFERENCE;
     state->lane_walk = kLibraryDefault;
     state->timing = 0;
     state->fetch_mode = TAY_FETCH_EVERY_RUN;
@@ -665,6 +668,18 @@ TAY_API int tay_set_tile_seers(TayState *state, unsigned seers) {
     state->tile_seers = seers;
     if (state->program)
         tay_gpu_program_set_tile_seers(state->program, seers);
+    return 1;
+}
+
+TAY_API int tay_set_see_order(TayState *state, int order) {
+    if (order != TAY_SEE_ORDER_REFERENCE && order != TAY_SEE_ORDER_BALANCED) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see order must be TAY_SEE_ORDER_REFERENCE or TAY_SEE_ORDER_BALANCED");
+        return 0;
+    }
+    if (state->see_order != order) {
+        state->see_order = order;
+        state->recompile = 1;
+    }
     return 1;
 }
 
