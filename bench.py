@@ -193,18 +193,19 @@ def roofline(flop_per_launch, launch_s, kernel, f_pair, note, traffic=None, **ex
     return out
 
 
-def timed_run(lib, sim, steps):
+def timed_run(lib, sim, steps, pass_timing=True):
     """Wall time of one tay_run of `steps` steps in the default fetch mode
     (it ends by copying every agent to host storage), the see and act pass
-    times, and the time of that final copy alone."""
+    times (HIP events around every pass launch, when pass_timing is set),
+    and the time of that final copy alone."""
     from tay_amd import api
     lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_EVERY_RUN)
-    lib.tay_set_pass_timing(sim.state, 1)
+    lib.tay_set_pass_timing(sim.state, 1 if pass_timing else 0)
     lib.tay_gpu_synchronize()
     t0 = time.perf_counter()
     sim.run(steps)
     wall = time.perf_counter() - t0
-    times = [lib.pass_time(sim.state, i) for i in range(len(sim.model.passes))]
+    times = [lib.pass_time(sim.state, i) for i in range(len(sim.model.passes))] if pass_timing else None
     t1 = time.perf_counter()
     lib.tay_fetch_agents(sim.state, None)
     fetch = time.perf_counter() - t1
@@ -389,9 +390,16 @@ def bench_c1(lib, args):
         twin.run(2)
         twin.run(steps)
         p_see = lib.pair_counts(twin.state, 0)[1]
+    # a step of this state takes about 0.1 ms, so the events that time the
+    # passes (a few microseconds each) would show in its wall time: steps/s
+    # comes from a run without them, the pass times from a second run of the
+    # same steps
     with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
         sim.run(2)
-        wall, ((see_ms, launches), _), fetch = timed_run(lib, sim, steps)
+        wall, _, fetch = timed_run(lib, sim, steps, pass_timing=False)
+    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
+        sim.run(2)
+        _, ((see_ms, launches), _), _ = timed_run(lib, sim, steps)
         p_in = int(sim.agents()["last_ncount"].sum())
     shares = {"inside": p_in / (p_see / steps)}
     f_pair = f_pair_of(PLAIN_F, shares)

@@ -351,6 +351,12 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
     const unsigned *src = sp->cols[sp->cur];
     unsigned *dst = sp->cols[sp->cur ^ 1];
 
+    /* A small space (one block sorts it) has its sort write the cell ranges
+     * too: the chain of a small step is bound by kernel-to-kernel latency,
+     * not by its bytes.  (Its box stays a many-block reduction: one block
+     * reading 10,000 positions took 20 us against 5 + 7 us for the box and
+     * geometry kernels.) */
+    const bool small = n <= kSmallSort;
     const unsigned box_blocks = blocks_for(n, kBuildBlock * 4, sp->box_blocks);
     k_box_partial<<<box_blocks, kBuildBlock, 0, s>>>(src, cap, n, sp->dims, sp->is_point, sp->box_partials);
     k_geometry<<<1, 64, 0, s>>>(sp->box_partials, box_blocks, sp->dims, sp->part[0], sp->part[1], sp->part[2],
@@ -362,8 +368,8 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
         src, cap, n, sp->dims, sp->geom, sp->keys[0], sp->ranges);
 
     const unsigned nblocks = (n + kRadixTile - 1) / kRadixTile;
-    if (n <= kSmallSort) {
-        k_small_sort<<<1, kRadixBlock, 0, s>>>(sp->keys[0], sp->vals[0], n);
+    if (small) {
+        k_small_sort<<<1, kRadixBlock, 0, s>>>(sp->keys[0], sp->vals[0], n, sp->dims, sp->geom, sp->ranges);
     } else {
         for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
             const unsigned shift = pass * kRadixBits;
@@ -379,8 +385,9 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
         }
     }
     /* kRadixPasses is even: the sorted keys and source indices are back in
-     * buffer 0 (the small sort works in place there) */
-    k_ranges<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], n, sp->dims, sp->geom, sp->ranges);
+     * buffer 0 (the small sort works in place there and writes the ranges) */
+    if (!small)
+        k_ranges<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], n, sp->dims, sp->geom, sp->ranges);
     k_permute<<<agent_blocks, kBuildBlock, 0, s>>>(sp->keys[0], sp->vals[0], n, sp->words, cap, sp->dims, sp->geom,
                                                    sp->ranges, src, dst);
     if (!launch_ok("z-grid sort"))

@@ -24,6 +24,13 @@ constexpr unsigned kRadixBlock = 1024;             /* threads of the histogram a
 constexpr unsigned kRadixWaves = kRadixBlock / 64;
 constexpr unsigned kRadixRounds = kRadixTile / kRadixBlock;   /* 64-key rounds per wavefront */
 constexpr unsigned kScanBlocks = 64;             /* segments of the histogram scan, at most one wavefront's lanes */
+/* A space of at most kSmallSort agents is sorted by one block (k_small_sort),
+ * kSmallBits per pass: keys and source indices (8 bytes per key) and the
+ * per-wave digit counts (16 x 512 words) fill the 160 KiB of LDS. */
+constexpr unsigned kSmallPerLane = 15;                          /* keys held by one lane */
+constexpr unsigned kSmallSort = kSmallPerLane * kRadixBlock;    /* 15,360 */
+constexpr unsigned kSmallBits = 9;
+constexpr unsigned kSmallRadix = 1u << kSmallBits;
 
 __device__ __forceinline__ unsigned compact2(unsigned v) {
     v &= 0x55555555u;
@@ -184,9 +191,12 @@ __device__ void grid_geometry(const float *lo, const float *hi, const float *par
         k = need > k ? need : k;
         product *= (double)count[a];
     }
-    double root = pow(product / (double)(max_cells ? max_cells : 1ull), 1.0 / (double)dims);
-    if (root > 2.0 && (unsigned long long)root - 1ull > k)
-        k = (unsigned long long)root - 1ull;
+    if (product > (double)max_cells) {
+        /* far beyond the cap: start the search near the answer */
+        double root = pow(product / (double)(max_cells ? max_cells : 1ull), 1.0 / (double)dims);
+        if (root > 2.0 && (unsigned long long)root - 1ull > k)
+            k = (unsigned long long)root - 1ull;
+    }
     for (;; ++k) {
         unsigned long long total = 1;
         bool fits = true;
@@ -521,9 +531,11 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const unsigned *_
     }
 }
 
-/* Stable LSD radix sort of at most kSmallSort keys by one block, all four
- * passes in LDS: one launch instead of four histogram, scan and scatter
- * rounds, for the small states whose step is a chain of short launches.
+/* Stable LSD radix sort of at most kSmallSort keys by one block, every pass
+ * in LDS, kSmallBits (9) per pass: one launch instead of four histogram,
+ * scan and scatter rounds, for the small states whose step is a chain of
+ * short launches (keys below 2^18, e.g. a 16^3-cell grid with two sub-cell
+ * bits, take two passes).
  * Wave w owns the contiguous slice [w * per, (w + 1) * per) of the current
  * order; per pass each lane keeps its keys in registers, ranks them inside
  * the wave round by round (ballot match, as k_radix_scatter), the digit
@@ -531,22 +543,34 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const unsigned *_
  * the LDS arrays at their new places; passes above the largest key's top
  * digit are skipped.  The source index of slot i starts as i.  Sorts
  * keys[0, n) in place and writes the source indices to vals. */
-constexpr unsigned kSmallSort = 16384;
-constexpr unsigned kSmallPerLane = kSmallSort / kRadixBlock;   /* keys held by one lane */
 
 __global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict__ keys,
-                                                            unsigned *__restrict__ vals, unsigned n) {
+                                                            unsigned *__restrict__ vals, unsigned n, int dims,
+                                                            const TayGridGeom *__restrict__ g,
+                                                            TayCellRange *__restrict__ ranges) {
     __shared__ unsigned lkeys[kSmallSort], lvals[kSmallSort];
-    __shared__ unsigned cnt[kRadixWaves][kRadix];
-    __shared__ unsigned wave_sum[kRadix / 64];
+    __shared__ unsigned cnt[kRadixWaves][kSmallRadix];
+    __shared__ unsigned wave_sum[kSmallRadix / 64];
     __shared__ unsigned wave_max_key[kRadixWaves];
     const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
     unsigned kmax = 0;
-    for (unsigned i = t; i < n; i += kRadixBlock) {
-        const unsigned k = keys[i];
-        lkeys[i] = k;
-        lvals[i] = i;
-        kmax = k > kmax ? k : kmax;
+    {
+        /* every load issued before the first is used: one memory latency */
+        unsigned k[kSmallPerLane];
+#pragma unroll
+        for (unsigned r = 0; r < kSmallPerLane; ++r) {
+            const unsigned i = t + r * kRadixBlock;
+            k[r] = i < n ? keys[i] : 0u;
+        }
+#pragma unroll
+        for (unsigned r = 0; r < kSmallPerLane; ++r) {
+            const unsigned i = t + r * kRadixBlock;
+            if (i < n) {
+                lkeys[i] = k[r];
+                lvals[i] = i;
+                kmax = k[r] > kmax ? k[r] : kmax;
+            }
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -562,26 +586,31 @@ __global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict
     const unsigned rounds = (n + kRadixWaves * 64u - 1u) / (kRadixWaves * 64u);   /* <= kSmallPerLane */
     const unsigned per = rounds * 64u;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
-        const unsigned shift = pass * kRadixBits;
+    for (unsigned pass = 0; pass * kSmallBits < 32u; ++pass) {
+        const unsigned shift = pass * kSmallBits;
         /* every digit from here up is zero: the remaining passes would keep
          * the order (stable), so the keys are sorted */
         if (pass > 0u && (kmax >> shift) == 0u)
             break;
-        for (unsigned k = t; k < kRadixWaves * kRadix; k += kRadixBlock)
+        for (unsigned k = t; k < kRadixWaves * kSmallRadix; k += kRadixBlock)
             (&cnt[0][0])[k] = 0;
         __syncthreads();
         unsigned key[kSmallPerLane], val[kSmallPerLane], pos[kSmallPerLane];
 #pragma unroll
         for (unsigned r = 0; r < kSmallPerLane; ++r) {
+            key[r] = val[r] = pos[r] = 0u;
+            /* rounds is the same for the whole block: the rounds past it
+             * are skipped, not ranked with every lane masked off */
+            if (r >= rounds)
+                continue;
             const unsigned idx = w * per + r * 64u + lane;
-            const bool valid = r < rounds && idx < n;
+            const bool valid = idx < n;
             key[r] = valid ? lkeys[idx] : 0u;
             val[r] = valid ? lvals[idx] : 0u;
-            const unsigned d = (key[r] >> shift) & (kRadix - 1);
+            const unsigned d = (key[r] >> shift) & (kSmallRadix - 1);
             unsigned long long m = __ballot(valid);
 #pragma unroll
-            for (unsigned b = 0; b < kRadixBits; ++b) {
+            for (unsigned b = 0; b < kSmallBits; ++b) {
                 const bool bit = (d >> b) & 1u;
                 const unsigned long long vote = __ballot(bit);
                 m &= bit ? vote : ~vote;
@@ -594,9 +623,9 @@ __global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        /* threads t < kRadix: digit t's start, then each wave's run start */
+        /* threads t < kSmallRadix: digit t's start, then each wave's run start */
         unsigned c = 0, inc = 0;
-        if (t < kRadix) {
+        if (t < kSmallRadix) {
             for (unsigned ww = 0; ww < kRadixWaves; ++ww)
                 c += cnt[ww][t];
             inc = wave_inclusive_sum(c, lane);
@@ -604,7 +633,7 @@ __global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict
                 wave_sum[w] = inc;
         }
         __syncthreads();
-        if (t < kRadix) {
+        if (t < kSmallRadix) {
             unsigned start = inc - c;
             for (unsigned ww = 0; ww < w; ++ww)
                 start += wave_sum[ww];
@@ -619,16 +648,29 @@ __global__ __launch_bounds__(kRadixBlock) void k_small_sort(unsigned *__restrict
         for (unsigned r = 0; r < kSmallPerLane; ++r) {
             const unsigned idx = w * per + r * 64u + lane;
             if (r < rounds && idx < n) {
-                const unsigned dst = cnt[w][(key[r] >> shift) & (kRadix - 1)] + pos[r];
+                const unsigned dst = cnt[w][(key[r] >> shift) & (kSmallRadix - 1)] + pos[r];
                 lkeys[dst] = key[r];
                 lvals[dst] = val[r];
             }
         }
         __syncthreads();
     }
+    /* the sorted keys and source indices, and (as k_ranges does from them)
+     * the [beg, end) of every occupied cell, into the table k_keys_clear
+     * cleared */
     for (unsigned i = t; i < n; i += kRadixBlock) {
         keys[i] = lkeys[i];
         vals[i] = lvals[i];
+        const unsigned key = cell_of_key(lkeys[i], dims, g);
+        const bool first = i == 0 || cell_of_key(lkeys[i - 1], dims, g) != key;
+        const bool last = i + 1 == n || cell_of_key(lkeys[i + 1], dims, g) != key;
+        if (first || last) {
+            TayCellRange *r = ranges + morton_to_cell(key, dims, g);
+            if (first)
+                r->beg = i;
+            if (last)
+                r->end = i + 1;
+        }
     }
 }
 

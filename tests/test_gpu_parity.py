@@ -107,7 +107,7 @@ def test_grid_geometry_bit_exact(gpu, oracle, models):
     assert np.array_equal(c1[:3], c2[:3]) and n1 == n2
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4097, 16384, 16385])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 4097, 15360, 15361, 16384, 16385])
 def test_ragged_sizes(gpu, oracle, models, n):
     recs = W.plain_agents(n, seed=100 + n, box=200.0)
     ga, oa, gp, op = run_pair(gpu, oracle, models, W.plain_model(30.0), recs, 2)
