@@ -544,11 +544,12 @@ void df_sort(TayDflock *f, const float *cols, hipStream_t s) {
     for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
         const unsigned in = pass & 1u, out = in ^ 1u;
         k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(f->keys[in], n, pass * kRadixBits, f->block_hist, nblocks,
-                                                     nullptr);
-        k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(f->block_hist, total, f->scan_sums);
-        k_scan_down<<<segs, kBuildBlock, 0, s>>>(f->block_hist, total, f->scan_sums);
+                                                     nullptr, nullptr);
+        k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(f->block_hist, total, f->scan_sums, nullptr, 0u);
+        k_scan_down<<<segs, kBuildBlock, 0, s>>>(f->block_hist, total, f->scan_sums, nullptr, 0u);
         k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(f->keys[in], f->vals[in], f->keys[out], f->vals[out], n,
-                                                        pass * kRadixBits, f->block_hist, nblocks, pass == 0, nullptr);
+                                                        pass * kRadixBits, f->block_hist, nblocks, pass == 0, nullptr,
+                                                        nullptr);
     }
     k_ranges_clear<<<blocks_for(f->max_cells, kBuildBlock, 4096), kBuildBlock, 0, s>>>(f->ranges, f->geom);
     k_ranges<<<agent_blocks, kBuildBlock, 0, s>>>(f->keys[0], n, 3, f->geom, f->ranges);

@@ -12,12 +12,15 @@
  * Sort, per step, for every instance at once:
  *   box       one wavefront per tile: min/max of its positions
  *   geometry  one wavefront per instance: reduce its tiles' boxes, then the
- *             single-space geometry
+ *             single-space geometry; the largest key any grid allows
  *   keys      per slot: the instance-local Morton key, padding 0xFFFFFFFF
- *   radix     4 stable passes over the key bytes, then 1 (or 2) stable passes
- *             over the instance index: instance-major, key-minor, and within
- *             a key the previous storage order -- exactly the order a sort
- *             of the instance alone produces
+ *   radix     stable passes over the key bytes on radix tiles that never
+ *             cross an instance, histograms laid out instance-major, so one
+ *             scan places every key inside its instance: key order, and
+ *             within a key the previous storage order -- exactly the order
+ *             a sort of the instance alone produces; a pass over digits
+ *             above the largest key the grids allow only copies
+ *             (radix_pass_idle)
  *   ranges    clear and fill each instance's table (absolute slots)
  *   permute   gather the columns, write the tag
  *   order     tiles bucketed by estimated cost (occupancy of the cell of the
@@ -48,7 +51,10 @@ constexpr unsigned kCostBuckets = 32;
 __global__ __launch_bounds__(kBuildBlock) void k_tile_box(const unsigned *__restrict__ cols, unsigned cap,
                                                           unsigned tiles, const unsigned *__restrict__ tile_inst,
                                                           const TayBatchInstance *__restrict__ inst, int dims,
-                                                          float *__restrict__ partials) {
+                                                          float *__restrict__ partials,
+                                                          unsigned *__restrict__ key_max) {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *key_max = 0u;      /* raised by k_batch_geometry */
     const unsigned t = blockIdx.x * (kBuildBlock / 64u) + (threadIdx.x >> 6);
     if (t >= tiles)
         return;
@@ -68,11 +74,14 @@ __global__ __launch_bounds__(kBuildBlock) void k_tile_box(const unsigned *__rest
 }
 
 /* One wavefront per instance: reduce its tiles' boxes, then lane 0 derives
- * the grid as tay_gpu_space_sort does. */
+ * the grid as tay_gpu_space_sort does, and raises key_max (zeroed by
+ * k_tile_box) to the largest key the grid allows: the Morton key of the
+ * last sub-cell on every axis, at least any agent's key. */
 __global__ __launch_bounds__(64) void k_batch_geometry(const float *__restrict__ partials,
                                                        const float *__restrict__ parts,
                                                        const TayBatchInstance *__restrict__ inst, int dims,
-                                                       int order_bits, TayGridGeom *__restrict__ geom) {
+                                                       int order_bits, TayGridGeom *__restrict__ geom,
+                                                       unsigned *__restrict__ key_max) {
     const unsigned b = blockIdx.x, lane = threadIdx.x;
     const TayBatchInstance in = inst[b];
     const unsigned t0 = in.off / 64u, t1 = (in.off + in.count + 63u) / 64u;
@@ -99,6 +108,11 @@ __global__ __launch_bounds__(64) void k_batch_geometry(const float *__restrict__
             lo[k] = hi[k] = 0.0f;   /* no agents: a one-cell grid that no tile reads */
     }
     grid_geometry(lo, hi, part, dims, in.max_cells, order_bits, geom + b);
+    const TayGridGeom &g = geom[b];
+    int last[3] = {0, 0, 0};
+    for (int k = 0; k < dims; ++k)
+        last[k] = ((g.count[k] - 1) << g.order_bits) | ((1 << g.order_bits) - 1);
+    atomicMax(key_max, morton(last[0], last[1], last[2], dims));
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_batch_keys(const unsigned *__restrict__ cols, unsigned cap,
@@ -353,6 +367,7 @@ struct TayGpuBatch {
     unsigned *block_hist;
     TaySortTile *sort_seg;
     unsigned *scan_sums;
+    unsigned *key_max;          /* largest key the grids allow this step (the word after scan_sums) */
     TayCellRange *ranges;
     unsigned *tile_bucket;
     unsigned *bucket_hist;
@@ -414,8 +429,9 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
     const unsigned agent_blocks = blocks_for(n, kBuildBlock, 0xFFFFFFFFu);
 
     k_tile_box<<<(b->tiles + kBuildBlock / 64u - 1u) / (kBuildBlock / 64u), kBuildBlock, 0, s>>>(
-        src, cap, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes);
-    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom);
+        src, cap, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes, b->key_max);
+    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom,
+                                                 b->key_max);
     k_batch_keys<<<agent_blocks, kBuildBlock, 0, s>>>(src, cap, n, b->tile_inst, b->inst, b->dims, b->geom,
                                                       b->keys[0]);
     /* every instance sorted in its own slot range: radix tiles never cross
@@ -427,11 +443,13 @@ bool batch_step(TayGpuBatch *b, hipStream_t s, bool record) {
     for (unsigned pass = 0; pass < kRadixPasses; ++pass) {
         const unsigned shift = pass * kRadixBits;
         const unsigned in = pass & 1u, out = in ^ 1u;
-        k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(b->keys[in], n, shift, b->block_hist, nblocks, b->sort_seg);
-        k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums);
-        k_scan_down<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums);
+        k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(b->keys[in], n, shift, b->block_hist, nblocks, b->sort_seg,
+                                                     b->key_max);
+        k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums, b->key_max, shift);
+        k_scan_down<<<segs, kBuildBlock, 0, s>>>(b->block_hist, total, b->scan_sums, b->key_max, shift);
         k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(b->keys[in], b->vals[in], b->keys[out], b->vals[out], n,
-                                                        shift, b->block_hist, nblocks, pass == 0, b->sort_seg);
+                                                        shift, b->block_hist, nblocks, pass == 0, b->sort_seg,
+                                                        b->key_max);
     }
     /* kRadixPasses is even: the sorted keys and source slots are back in buffer 0 */
     const unsigned f = 0;
@@ -637,7 +655,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
     b->parts = b->boxes = nullptr;
     b->geom = nullptr;
     b->keys[0] = b->keys[1] = b->vals[0] = b->vals[1] = nullptr;
-    b->block_hist = b->scan_sums = nullptr;
+    b->block_hist = b->scan_sums = b->key_max = nullptr;
     b->sort_seg = nullptr;
     b->ranges = nullptr;
     b->tile_bucket = b->bucket_hist = b->bucket_off = b->order = b->queues = nullptr;
@@ -686,7 +704,7 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
               device_alloc((void **)&b->vals[1], 4ull * slots, "sort values") &&
               device_alloc((void **)&b->block_hist, 4ull * kRadix * nblocks, "radix histogram") &&
               device_alloc((void **)&b->sort_seg, sizeof(TaySortTile) * nblocks, "sort tiles") &&
-              device_alloc((void **)&b->scan_sums, 4ull * kScanBlocks, "scan sums") &&
+              device_alloc((void **)&b->scan_sums, 4ull * (kScanBlocks + 1), "scan sums") &&
               device_alloc((void **)&b->ranges, sizeof(TayCellRange) * cells, "cell tables") &&
               device_alloc((void **)&b->tile_bucket, 4ull * b->tiles, "tile buckets") &&
               device_alloc((void **)&b->bucket_hist, 4ull * kCostBuckets, "bucket histogram") &&
@@ -705,6 +723,8 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
                      "upload part sizes") &&
               hip_ok(hipStreamSynchronize(s), "hipStreamSynchronize") &&
               hip_ok(hipModuleLoadData(&b->module, code.data()), "hipModuleLoadData");
+    if (ok)
+        b->key_max = b->scan_sums + kScanBlocks;
     int cus = 0;
     ok = ok && hip_ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
     b->passes.resize(passes_count);
@@ -1047,8 +1067,9 @@ bool instance_geometry(TayGpuBatch *b, std::vector<TayGridGeom> &geom) {
         return false;
     if (b->tiles)
         k_tile_box<<<(b->tiles + kBuildBlock / 64u - 1u) / (kBuildBlock / 64u), kBuildBlock, 0, s>>>(
-            b->cols[b->cur], b->slots, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes);
-    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom);
+            b->cols[b->cur], b->slots, b->tiles, b->tile_inst, b->inst, b->dims, b->boxes, b->key_max);
+    k_batch_geometry<<<b->instances, 64, 0, s>>>(b->boxes, b->parts, b->inst, b->dims, b->order_bits, b->geom,
+                                                 b->key_max);
     geom.resize(b->instances);
     return launch_ok("instance geometry") &&
            hip_ok(hipMemcpyAsync(geom.data(), b->geom, sizeof(TayGridGeom) * b->instances, hipMemcpyDeviceToHost,

@@ -377,11 +377,13 @@ TAY_API int tay_gpu_space_sort(TayGpuSpace *sp) {
             const unsigned total = kRadix * nblocks;
             const unsigned segs =
                 total < kScanBlocks * kBuildBlock ? (total + kBuildBlock - 1) / kBuildBlock : kScanBlocks;
-            k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], n, shift, sp->block_hist, nblocks, nullptr);
-            k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
-            k_scan_down<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums);
+            k_radix_hist<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], n, shift, sp->block_hist, nblocks, nullptr,
+                                                         nullptr);
+            k_scan_reduce<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums, nullptr, shift);
+            k_scan_down<<<segs, kBuildBlock, 0, s>>>(sp->block_hist, total, sp->scan_sums, nullptr, shift);
             k_radix_scatter<<<nblocks, kRadixBlock, 0, s>>>(sp->keys[in], sp->vals[in], sp->keys[out], sp->vals[out],
-                                                            n, shift, sp->block_hist, nblocks, pass == 0, nullptr);
+                                                            n, shift, sp->block_hist, nblocks, pass == 0, nullptr,
+                                                            nullptr);
         }
     }
     /* kRadixPasses is even: the sorted keys and source indices are back in

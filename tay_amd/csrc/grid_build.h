@@ -360,9 +360,21 @@ __device__ __forceinline__ TaySortTile sort_tile(const TaySortTile *__restrict__
 }
 
 /* Digit histogram of one tile of at most 4096 keys per block. */
+/* The radix pass at `shift` leaves the order as it is when every real key
+ * is below 2^shift - 1: their digits from `shift` up are zero, and a
+ * padding key (all ones, a batch's empty slots) already sorts after them on
+ * its lower bits.  key_max is the largest real key of the sort (nullptr:
+ * no pass is skipped); the first pass always runs. */
+__device__ __forceinline__ bool radix_pass_idle(const unsigned *key_max, unsigned shift) {
+    return key_max && shift && *key_max < (1u << shift) - 1u;
+}
+
 __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const unsigned *__restrict__ keys, unsigned n,
                                                             unsigned shift, unsigned *__restrict__ hist,
-                                                            unsigned nblocks, const TaySortTile *__restrict__ seg) {
+                                                            unsigned nblocks, const TaySortTile *__restrict__ seg,
+                                                            const unsigned *__restrict__ key_max) {
+    if (radix_pass_idle(key_max, shift))
+        return;
     __shared__ unsigned h[kRadix];
     if (threadIdx.x < kRadix)
         h[threadIdx.x] = 0;
@@ -392,7 +404,10 @@ __device__ __forceinline__ unsigned segment_begin(unsigned s, unsigned total, un
 }
 
 __global__ __launch_bounds__(kBuildBlock) void k_scan_reduce(const unsigned *__restrict__ hist, unsigned total,
-                                                             unsigned *__restrict__ sums) {
+                                                             unsigned *__restrict__ sums,
+                                                             const unsigned *__restrict__ key_max, unsigned shift) {
+    if (radix_pass_idle(key_max, shift))
+        return;
     __shared__ unsigned s[kBuildBlock / 64];
     const unsigned beg = segment_begin(blockIdx.x, total, gridDim.x);
     const unsigned end = segment_begin(blockIdx.x + 1, total, gridDim.x);
@@ -414,7 +429,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_scan_reduce(const unsigned *__r
 
 
 __global__ __launch_bounds__(kBuildBlock) void k_scan_down(unsigned *__restrict__ hist, unsigned total,
-                                                           const unsigned *__restrict__ sums) {
+                                                           const unsigned *__restrict__ sums,
+                                                           const unsigned *__restrict__ key_max, unsigned shift) {
+    if (radix_pass_idle(key_max, shift))
+        return;
     __shared__ unsigned s[kBuildBlock / 64];
     const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const unsigned beg = segment_begin(blockIdx.x, total, gridDim.x);
@@ -456,13 +474,23 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const unsigned *_
                                                                unsigned *__restrict__ vals_out, unsigned n,
                                                                unsigned shift, const unsigned *__restrict__ offsets,
                                                                unsigned nblocks, int identity_vals,
-                                                               const TaySortTile *__restrict__ seg) {
+                                                               const TaySortTile *__restrict__ seg,
+                                                               const unsigned *__restrict__ key_max) {
     __shared__ unsigned wave_hist[kRadixWaves][kRadix];
     __shared__ unsigned lbase[kRadixWaves][kRadix];   /* next tile slot of (wave, digit) */
     __shared__ unsigned delta[kRadix];                /* output slot minus tile slot of a digit */
     __shared__ unsigned wave_sum[kRadix / 64];
     __shared__ unsigned lkeys[kRadixTile], lvals[kRadixTile];
     const unsigned t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    if (radix_pass_idle(key_max, shift)) {
+        /* an idle pass still moves the tile to the other buffer */
+        const TaySortTile tile = sort_tile(seg, blockIdx.x, n, nblocks);
+        for (unsigned r = t; r < tile.count; r += kRadixBlock) {
+            keys_out[tile.start + r] = keys_in[tile.start + r];
+            vals_out[tile.start + r] = vals_in[tile.start + r];
+        }
+        return;
+    }
     for (unsigned k = t; k < kRadixWaves * kRadix; k += kRadixBlock)
         (&wave_hist[0][0])[k] = 0;
     __syncthreads();

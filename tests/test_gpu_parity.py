@@ -93,6 +93,16 @@ def test_flocking_clustered_bit_exact(gpu, oracle, models):
     assert gp == op
 
 
+def test_flocking_70k_bit_exact(gpu, oracle, models):
+    """70,000 clustered flocking agents: the seer count picks 32-seer tiles
+    (64-seer tiles would give the SIMDs fewer than two wavefronts each), the
+    radix sort runs its multi-kernel path; bit-exact over two steps."""
+    recs = W.flock_agents(70000, seed=15, clusters=16, sigma=30.0, box=1000.0)
+    ga, oa, gp, op = run_pair(gpu, oracle, models, W.flock_model(25.0), recs, 2)
+    assert_bitwise(ga, oa)
+    assert gp == op
+
+
 def test_grid_geometry_bit_exact(gpu, oracle, models):
     recs = W.flock_agents(5000, seed=6, clusters=4, sigma=40.0, box=800.0)
     model = W.flock_model(25.0)
