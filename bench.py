@@ -96,9 +96,13 @@ SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of 
 # accumulators (8), the seen side reads p, v, density and pressure (8);
 # density, the seer reads p and density (4) and writes density (1), the seen
 # side reads p (3).
-SPH_TRAFFIC = {"particles": 2_000_000, "kernel": "see kernels of commit c7e55b7", "order": "reference",
-               "forces": int((69859.7 * 2.000 + 46875.0 * 1.000) * 1024),
-               "density": int((19950.6 * 2.000 + 7812.5 * 1.000) * 1024)}
+SPH_TRAFFIC = {"particles": 2_000_000,
+               "reference": {"kernel": "see kernels of commit c7e55b7",
+                             "forces": int((69859.7 * 2.000 + 46875.0 * 1.000) * 1024),
+                             "density": int((19950.6 * 2.000 + 7812.5 * 1.000) * 1024)},
+               "balanced": {"kernel": "see kernels of commit aaf7979",
+                            "forces": int((128419.3 * 2.000 + 124785.2 * 1.000) * 1024),
+                            "density": int((19950.9 * 2.000 + 7812.5 * 1.000) * 1024)}}
 B_SPH_FORCES_PER_PARTICLE = 4 * (15 + 8 + 8)
 B_SPH_DENSITY_PER_PARTICLE = 4 * (4 + 1 + 3)
 
@@ -601,7 +605,8 @@ def bench_sph(lib, args):
         inside.append(((ball + len(a)) / (box + len(a)), ball / box))
     share_d = 0.5 * (inside[0][0] + inside[1][0])
     share_f = 0.5 * (inside[0][1] + inside[1][1])
-    measured = len(recs) == SPH_TRAFFIC["particles"] and args.sph_see_order == SPH_TRAFFIC["order"]
+    measured = len(recs) == SPH_TRAFFIC["particles"]
+    traffic = SPH_TRAFFIC[args.sph_see_order]
     f_d = f_pair_of(SPH_DENSITY_F, {"inside": share_d})
     f_f = f_pair_of(SPH_FORCES_F, {"inside": share_f})
     out = {"workload": "SPH dam break, %d particles" % len(recs), "particles": len(recs), "h": 1.0, "dt": 1e-3,
@@ -613,12 +618,12 @@ def bench_sph(lib, args):
                                 "the force pass, the longer of the two see kernels; share of box pairs inside h "
                                 "measured on the window's first and last particles; traffic from rocprofv3 "
                                 "FETCH_SIZE/WRITE_SIZE (profiles/see_sph2M_pmc.md)",
-                                traffic=SPH_TRAFFIC["forces"] if measured else None,
+                                traffic=traffic["forces"] if measured else None,
                                 traffic_bytes_algorithmic=B_SPH_FORCES_PER_PARTICLE * len(recs),
                                 branch_shares={"inside": share_f},
                                 density=roofline(pairs[0] / steps * f_d, see[0] / 1e3 / steps,
                                                  "see (tay_pass_0, sph_density)", f_d, "",
-                                                 traffic=SPH_TRAFFIC["density"] if measured else None,
+                                                 traffic=traffic["density"] if measured else None,
                                                  traffic_bytes_algorithmic=B_SPH_DENSITY_PER_PARTICLE * len(recs),
                                                  branch_shares={"inside": share_d})),
            other + "_see_order": {"steps_per_s": steps / o_wall, "ms_per_step": o_wall * 1e3 / steps,
