@@ -74,6 +74,27 @@ typedef enum TaySeeOrder {
 } TaySeeOrder;
 int tay_set_see_order(TayState *state, int order);
 
+/* Splits each seer of see pass `pass_index` (its place among the state's
+ * passes) over `lanes` (1, 2, 4 or 8; 1 undoes the split) lanes of the see
+ * kernel, for a model whose see function only adds up its pairs'
+ * contributions.  Lane k of a seer takes the candidates at positions
+ * p = k (mod lanes) of its lane group's sweep (cells x, then y, then z,
+ * each in storage order), in that order, into a partial record: partial 0
+ * is the seer itself, partial k > 0 a copy emptied by `clear`; at the end
+ * of the tile the partials merge by `combine(a, b)` (a += b) in a fixed
+ * tree: pairs (0 + 1), (2 + 3), ..., then pairs of those, ((0 + 1) + (2 +
+ * 3)) for 4 lanes.  A wavefront then holds 64 / lanes seers and its compact
+ * rounds run each seer's pairs on `lanes` lanes: the same pairs and counts,
+ * float sums in that fixed order.  A lane group whose cell box is too wide
+ * halves down to single seers, whatever tay_set_lane_walk says (a lane walk
+ * would use one lane of each seer).  The function pointers are the
+ * CPU callbacks (tay_add_see's convention; unused by this library), the
+ * names the model functions to compile.  Such a pass runs in the reference
+ * see order whatever tay_set_see_order says.  Takes effect at the next
+ * tay_run.  No reference counterpart. */
+int tay_set_see_split(TayState *state, unsigned pass_index, unsigned lanes, void (*clear)(void *),
+                      const char *clear_name, void (*combine)(void *, void *), const char *combine_name);
+
 /* The device space of a group, valid while the simulation runs (NULL before
  * the first tay_run and for groups not on the GPU). */
 TayGpuSpace *tay_get_gpu_space(TayState *state, TayGroup *group);

@@ -155,6 +155,9 @@ typedef struct TayGpuPassDesc {
     unsigned context_size;      /* bytes copied by tay_gpu_program_set_context */
     int see_order;              /* see passes over a Z-grid: 0 the reference sweep order, 1 the balanced
                                    order (TaySeeOrder, tay_ext.h) */
+    unsigned split_lanes;       /* see passes: 1, or 2, 4 or 8 lanes per seer (tay_set_see_split) */
+    const char *split_clear;    /* with split_lanes > 1: the model's clear and combine functions */
+    const char *split_combine;
 } TayGpuPassDesc;
 
 /* Composes the model sources and one generated kernel per pass into a single
@@ -181,6 +184,9 @@ typedef struct TayGpuPassShape {
     int skip_self;              /* 1 when seer and seen are one group and self_see is 0 */
     unsigned context_size;      /* bytes of the pass context */
     int see_order;              /* as TayGpuPassDesc::see_order */
+    unsigned split_lanes;       /* as TayGpuPassDesc::split_lanes, split_clear, split_combine */
+    const char *split_clear;
+    const char *split_combine;
 } TayGpuPassShape;
 
 /* Composes and compiles a program to a gfx950 code object without loading
@@ -199,23 +205,24 @@ int tay_gpu_program_run_pass(TayGpuProgram *program, unsigned pass_index);
 /* Seers per lane group when a see tile starts: 64 (the default), 32, 16 or
  * 8.  A wavefront always owns 64 seers; smaller groups walk tighter cell
  * boxes one after another, which pays where agents are sparse.  Results do
- * not depend on it.  No reference counterpart (a tuning knob of this
- * library). */
+ * not depend on it in the reference see order; in the balanced order it is
+ * part of the order (see_order of TayGpuPassDesc).  No reference
+ * counterpart (a tuning knob of this library). */
 int tay_gpu_program_set_tile_width(TayGpuProgram *program, unsigned width);
 
 /* A lane group of at most `width` seers (0 to 64, default 64) whose cell
  * box is still too wide for one shared sweep walks per lane (every seer its
  * own cells, candidates one by one, all members at once) instead of
  * splitting into smaller groups swept one after another; 0 always splits.
- * Pays where agents are sparse.  Results do not depend on it.  No reference
- * counterpart. */
+ * Pays where agents are sparse.  As the tile width, part of the order only
+ * in the balanced see order.  No reference counterpart. */
 int tay_gpu_program_set_lane_walk(TayGpuProgram *program, unsigned width);
 
 /* Seers per see tile (one wavefront owns a tile): 64, 32 or 16, or 0 (the
  * default) to choose by the pass's seer count: 64 while that gives every
  * SIMD two wavefronts, fewer for smaller states, whose passes are bound by
- * the sweep of their few tiles.  Results do not depend on it.  No
- * reference counterpart. */
+ * the sweep of their few tiles.  As the tile width, part of the order only
+ * in the balanced see order.  No reference counterpart. */
 int tay_gpu_program_set_tile_seers(TayGpuProgram *program, unsigned seers);
 
 /* Pairs that passed the box test (agent_see calls) since the last reset;

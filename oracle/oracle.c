@@ -140,6 +140,10 @@ typedef struct TayPass {
     /* GPU Z-grid seen spaces: see order and tile layout (the state's) */
     int see_order;
     unsigned tile_seers, tile_width, lane_walk;
+    /* split see pass (tay_set_see_split): lanes per seer, partial clear and merge */
+    unsigned split_lanes;
+    void (*split_clear)(void *);
+    void (*split_combine)(void *, void *);
 } TayPass;
 
 typedef struct TayState {
@@ -958,6 +962,29 @@ int tay_set_see_order(TayState *state, int order) {
     return 1;
 }
 
+int tay_set_see_split(TayState *state, unsigned pass_index, unsigned lanes, void (*clear)(void *),
+                      const char *clear_name, void (*combine)(void *, void *), const char *combine_name) {
+    (void)clear_name;
+    (void)combine_name;
+    if (pass_index >= state->passes_count || state->passes[pass_index].type != PASS_SEE) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see split: no see pass at that index");
+        return 0;
+    }
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see split: lanes per seer must be 1, 2, 4 or 8");
+        return 0;
+    }
+    if (lanes > 1 && (!clear || !combine)) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see split: clear and combine functions must be given");
+        return 0;
+    }
+    TayPass *p = state->passes + pass_index;
+    p->split_lanes = lanes;
+    p->split_clear = clear;
+    p->split_combine = combine;
+    return 1;
+}
+
 int tay_set_tile_seers(TayState *state, unsigned seers) {
     if (seers != 0 && seers != 64 && seers != 32 && seers != 16) {
         set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "seers per tile must be 0, 64, 32 or 16");
@@ -1249,6 +1276,76 @@ static void gpu_z_grid_balanced_sweep(TayPass *p, unsigned m0, unsigned m1, cons
     }
 }
 
+/* One pair for a seer record that may be a partial (split pass): seer index
+ * i (for the self test) and seen index j, with the box test of the groups'
+ * agent kinds and the self rule of the pairing loops (tay/space.c:106-238). */
+static void pair_one(TayPass *p, char *rec, unsigned i, unsigned j, PairCounts *c) {
+    if (p->seer_group == p->seen_group && !p->self_see && i == j)
+        return;
+    const char *b = p->seen_group->storage + (size_t)p->seen_group->agent_size * j;
+    const int dims = p->seer_group->space.dims;
+    const float *r = p->radii.arr;
+    const int sp = p->seer_group->is_point, np = p->seen_group->is_point;
+    ++c->broad;
+    int ok;
+    if (sp)
+        ok = np ? narrow_point_point(POS(rec), POS(b), r, dims) : narrow_point_nonpoint(POS(rec), POS(b), MAXC(b), r, dims);
+    else
+        ok = np ? narrow_nonpoint_point(POS(rec), MAXC(rec), POS(b), r, dims)
+                : narrow_nonpoint_nonpoint(POS(rec), MAXC(rec), POS(b), MAXC(b), r, dims);
+    if (!ok)
+        return;
+    ++c->narrow;
+    p->see(rec, (void *)b, p->context);
+}
+
+/* Split see pass (tay_set_see_split, K = split_lanes): the kernel's lanes
+ * of a seer share its lane group's sweep, candidates at sweep positions
+ * k (mod K) going to partial k; rec[m * K + k] is member m's partial k
+ * (partial 0 the seer's own record). */
+static void gpu_z_grid_split_sweep(TayPass *p, unsigned m0, unsigned m1, const int *lo, const int *hi, char **rec,
+                                   PairCounts *c) {
+    const Space *s = &p->seen_group->space;
+    const unsigned K = p->split_lanes;
+    unsigned pos = 0;
+    Int4 idx = {{0, 0, 0, 0}};
+    for (idx.arr[0] = lo[0]; idx.arr[0] <= hi[0]; ++idx.arr[0])
+        for (idx.arr[1] = lo[1]; idx.arr[1] <= hi[1]; ++idx.arr[1])
+            for (idx.arr[2] = lo[2]; idx.arr[2] <= hi[2]; ++idx.arr[2]) {
+                const CellRange *cr = s->ranges + row_major(idx, s->cell_counts, s->dims);
+                for (unsigned j = cr->beg; j < cr->end; ++j, ++pos)
+                    for (unsigned i = m0; i < m1; ++i)
+                        pair_one(p, rec[(i - m0) * K + pos % K], i, j, c);
+            }
+}
+
+/* The per-seer walk of a split pass: each member walks the cells of its own
+ * box (gpu_z_grid_range) in x, y, z order, the cells of row (x, y) going to
+ * partial k when the row's index in the box, (x - lo.x) * ny + (y - lo.y),
+ * is k mod K. */
+static void gpu_z_grid_split_walk(TayPass *p, unsigned m0, unsigned m1, char **rec, PairCounts *c) {
+    const Space *s = &p->seen_group->space;
+    const unsigned K = p->split_lanes;
+    for (unsigned i = m0; i < m1; ++i) {
+        const char *a = p->seer_group->storage + (size_t)p->seer_group->agent_size * i;
+        const float *lo_pos = POS(a);
+        const float *hi_pos = p->seer_group->is_point ? lo_pos : MAXC(a);
+        Int4 lo, hi, idx = {{0, 0, 0, 0}};
+        gpu_z_grid_range(s, lo_pos, hi_pos, p->radii.arr, s->dims, &lo, &hi);
+        const unsigned ny = (unsigned)(hi.arr[1] - lo.arr[1] + 1);
+        for (idx.arr[0] = lo.arr[0]; idx.arr[0] <= hi.arr[0]; ++idx.arr[0])
+            for (idx.arr[1] = lo.arr[1]; idx.arr[1] <= hi.arr[1]; ++idx.arr[1]) {
+                const unsigned row = (unsigned)(idx.arr[0] - lo.arr[0]) * ny + (unsigned)(idx.arr[1] - lo.arr[1]);
+                char *r = rec[(i - m0) * K + row % K];
+                for (idx.arr[2] = lo.arr[2]; idx.arr[2] <= hi.arr[2]; ++idx.arr[2]) {
+                    const CellRange *cr = s->ranges + row_major(idx, s->cell_counts, s->dims);
+                    for (unsigned j = cr->beg; j < cr->end; ++j)
+                        pair_one(p, r, i, j, c);
+                }
+            }
+    }
+}
+
 /* Seers [beg, end) of the seer group, whole tiles of S, out of `seers`. */
 static void gpu_z_grid_see_tiles(TayPass *p, unsigned beg, unsigned end, unsigned seers, unsigned S,
                                  PairCounts *c) {
@@ -1264,8 +1361,23 @@ static void gpu_z_grid_see_tiles(TayPass *p, unsigned beg, unsigned end, unsigne
         limit *= (unsigned)ceilf((2.0f * r[k] + ext[k]) / s->cell_sizes.arr[k]) + 3u;
     }
     const unsigned width0 = p->tile_width < S ? p->tile_width : S;
+    /* a split pass: partial records 1..K-1 of the tile's seers */
+    const unsigned K = p->split_lanes > 1u ? p->split_lanes : 1u;
+    const size_t asize = (size_t)sg->agent_size;
+    char *parts = K > 1u ? (char *)malloc(asize * S * (K - 1u)) : NULL;
+    char **rec = K > 1u ? (char **)malloc(sizeof(char *) * S * K) : NULL;
     for (unsigned i0 = beg; i0 < end; i0 += S) {
         const unsigned n_active = seers - i0 < S ? seers - i0 : S;
+        if (K > 1u)
+            for (unsigned m = 0; m < n_active; ++m) {
+                rec[m * K] = sg->storage + asize * (i0 + m);
+                for (unsigned k = 1; k < K; ++k) {
+                    char *q = parts + asize * (m * (K - 1u) + (k - 1u));
+                    memcpy(q, rec[m * K], asize);
+                    p->split_clear(q);
+                    rec[m * K + k] = q;
+                }
+            }
         unsigned gbeg = 0, width = width0;
         while (gbeg < n_active) {
             const unsigned m0 = i0 + gbeg, m1 = i0 + (gbeg + width < n_active ? gbeg + width : n_active);
@@ -1291,15 +1403,30 @@ static void gpu_z_grid_see_tiles(TayPass *p, unsigned beg, unsigned end, unsigne
                 continue;
             }
             if (cells > limit && width > 1u) {
-                gpu_z_grid_see_seen_one(p, seer_slice_of(p, m0, m1), c);
+                if (K > 1u)
+                    gpu_z_grid_split_walk(p, m0, m1, rec + (m0 - i0) * K, c);
+                else
+                    gpu_z_grid_see_seen_one(p, seer_slice_of(p, m0, m1), c);
             } else if (hi[0] >= lo[0] && hi[1] >= lo[1] && hi[2] >= lo[2]) {
-                gpu_z_grid_balanced_sweep(p, m0, m1, lo, hi, c);
+                if (K > 1u)
+                    gpu_z_grid_split_sweep(p, m0, m1, lo, hi, rec + (m0 - i0) * K, c);
+                else
+                    gpu_z_grid_balanced_sweep(p, m0, m1, lo, hi, c);
             }
             gbeg += width;
             while (width < width0 && (gbeg & (2u * width - 1u)) == 0u)
                 width <<= 1;
         }
+        /* merge the partials in the kernel's tree: (0 + 1), (2 + 3), ...,
+         * then pairs of those */
+        if (K > 1u)
+            for (unsigned m = 0; m < n_active; ++m)
+                for (unsigned step = 1; step < K; step <<= 1)
+                    for (unsigned k = 0; k < K; k += 2u * step)
+                        p->split_combine(rec[m * K + k], rec[m * K + k + step]);
     }
+    free(parts);
+    free(rec);
 }
 
 /* -------------------------------------------------------------- see/act */
@@ -1334,10 +1461,12 @@ static void see_task(void *task_p, int thread_i, PairCounts *c) {
     int dims = ss->dims < p->seen_group->space.dims ? ss->dims : p->seen_group->space.dims;
     unsigned beg, end;
     unsigned seers = p->seer_limit && p->seer_limit < ss->count ? p->seer_limit : ss->count;
-    if (p->see_order == 1 && p->seen_group->space.type == TAY_GPU_Z_GRID && ss->type != TAY_CPU_GRID &&
+    const int split = p->split_lanes > 1u;
+    if ((p->see_order == 1 || split) && p->seen_group->space.type == TAY_GPU_Z_GRID && ss->type != TAY_CPU_GRID &&
         ss->type != TAY_CPU_Z_GRID) {
-        /* balanced order: each thread takes whole tiles */
-        const unsigned S = p->tile_seers ? p->tile_seers : auto_tile_seers(ss->count);
+        /* balanced order or split pass: each thread takes whole tiles; a
+         * split pass has 64 / lanes seers per tile and the sweep order */
+        const unsigned S = split ? 64u / p->split_lanes : p->tile_seers ? p->tile_seers : auto_tile_seers(ss->count);
         unsigned tb, te;
         thread_range((seers + S - 1u) / S, thread_i, &tb, &te);
         gpu_z_grid_see_tiles(p, tb * S, te * S < seers ? te * S : seers, seers, S, c);

@@ -53,6 +53,12 @@ int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
  * SIMDs), starting lane-group width, lane-walk width).  The GPU library
  * exports the same calls (include/tay_ext.h). */
 int tay_set_see_order(TayState *state, int order);
+/* A split see pass (as the GPU library's call, include/tay_ext.h): every
+ * seer's pairs go to `lanes` (1, 2, 4 or 8) partial records by their
+ * position in the lane group's sweep, merged at the end of the tile; the
+ * oracle calls the CPU callbacks clear and combine. */
+int tay_set_see_split(TayState *state, unsigned pass_index, unsigned lanes, void (*clear)(void *),
+                      const char *clear_name, void (*combine)(void *, void *), const char *combine_name);
 int tay_set_tile_seers(TayState *state, unsigned seers);
 int tay_set_tile_width(TayState *state, unsigned width);
 int tay_set_lane_walk(TayState *state, unsigned width);

@@ -58,6 +58,22 @@ void flock_see(global FlockAgent *a, global FlockAgent *b, constant FlockSeeCont
     }
 }
 
+void flock_see_clear(global FlockAgent *a) {
+    a->separation = float4_null();
+    a->alignment = float4_null();
+    a->cohesion = float4_null();
+    a->separation_count = 0;
+    a->cohesion_count = 0;
+}
+
+void flock_see_combine(global FlockAgent *a, global FlockAgent *b) {
+    a->separation = float4_add(a->separation, b->separation);
+    a->alignment = float4_add(a->alignment, b->alignment);
+    a->cohesion = float4_add(a->cohesion, b->cohesion);
+    a->separation_count += b->separation_count;
+    a->cohesion_count += b->cohesion_count;
+}
+
 void flock_act(global FlockAgent *a, constant FlockActContext *c) {
     float4 acc = float4_null();
     if (a->separation_count)

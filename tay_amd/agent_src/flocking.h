@@ -98,5 +98,11 @@ void boid_see(global Boid *a, global Boid *b, constant BoidSeeContext *c);
 void boid_act(global Boid *a, constant BoidActContext *c);
 void flock_see(global FlockAgent *a, global FlockAgent *b, constant FlockSeeContext *c);
 void flock_act(global FlockAgent *a, constant FlockActContext *c);
+/* For a split see pass (tay_set_see_split): flock_see only adds to the
+ * sums and counts, so a partial of a seer starts with them emptied
+ * (flock_see_clear) and partials merge by adding them (flock_see_combine,
+ * a += b). */
+void flock_see_clear(global FlockAgent *a);
+void flock_see_combine(global FlockAgent *a, global FlockAgent *b);
 void smooth_see(global SmoothAgent *a, global SmoothAgent *b, constant SmoothSeeContext *c);
 void smooth_act(global SmoothAgent *a, constant SmoothActContext *c);

@@ -32,6 +32,32 @@ void sph_forces(global SphParticle *a, global SphParticle *b, constant SphContex
     a->viscosity_accum = float4_add(a->viscosity_accum, float4_mul_scalar(dv, c->viscosity * w / b->density));
 }
 
+void sph_density_clear(global SphParticle *a) {
+    a->density = 0.0f;
+}
+
+void sph_density_combine(global SphParticle *a, global SphParticle *b) {
+    a->density += b->density;
+}
+
+void sph_forces_clear(global SphParticle *a) {
+    a->pressure_accum.x = 0.0f;
+    a->pressure_accum.y = 0.0f;
+    a->pressure_accum.z = 0.0f;
+    a->viscosity_accum.x = 0.0f;
+    a->viscosity_accum.y = 0.0f;
+    a->viscosity_accum.z = 0.0f;
+}
+
+void sph_forces_combine(global SphParticle *a, global SphParticle *b) {
+    a->pressure_accum.x += b->pressure_accum.x;
+    a->pressure_accum.y += b->pressure_accum.y;
+    a->pressure_accum.z += b->pressure_accum.z;
+    a->viscosity_accum.x += b->viscosity_accum.x;
+    a->viscosity_accum.y += b->viscosity_accum.y;
+    a->viscosity_accum.z += b->viscosity_accum.z;
+}
+
 void sph_integrate(global SphParticle *a, constant SphContext *c) {
     float4 f = float4_add(a->pressure_accum, float4_mul_scalar(a->viscosity_accum, c->dynamic_viscosity));
     float4 acc = float4_div_scalar(f, a->density);

@@ -50,6 +50,15 @@ void sph_density(global SphParticle *a, global SphParticle *b, constant SphConte
 void sph_pressure(global SphParticle *a, constant SphContext *c);
 void sph_forces(global SphParticle *a, global SphParticle *b, constant SphContext *c);
 void sph_integrate(global SphParticle *a, constant SphContext *c);
+/* For split see passes (tay_set_see_split): sph_density only adds to the
+ * density and sph_forces only to the x, y, z of the two accumulators (it
+ * reads the seer's pressure, which a partial keeps), so a partial starts
+ * with those emptied (*_clear) and partials merge by adding them
+ * (*_combine, a += b); the accumulators' w is left alone. */
+void sph_density_clear(global SphParticle *a);
+void sph_density_combine(global SphParticle *a, global SphParticle *b);
+void sph_forces_clear(global SphParticle *a);
+void sph_forces_combine(global SphParticle *a, global SphParticle *b);
 
 /* The reference plug-in's own pass functions, under its names
  * (m_sph/agent.h:34-38, m_sph/main.c:70-74):

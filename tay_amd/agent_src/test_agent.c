@@ -88,3 +88,37 @@ void plain_act(global PlainAgent *a, constant ActContext *c) {
     a->p.z += a->v.z;
     reflect_point(&a->p, &a->v, c->min, c->max);
 }
+
+void plain_see_clear(global PlainAgent *a) {
+    a->nsum.x = 0.0f;
+    a->nsum.y = 0.0f;
+    a->nsum.z = 0.0f;
+    a->ncount = 0u;
+}
+
+void plain_see_combine(global PlainAgent *a, global PlainAgent *b) {
+    a->nsum.x += b->nsum.x;
+    a->nsum.y += b->nsum.y;
+    a->nsum.z += b->nsum.z;
+    a->ncount += b->ncount;
+}
+
+void agent_see_clear(global Agent *a) {
+    a->b_buffer = float4_null();
+    a->b_buffer_count = 0;
+}
+
+void agent_see_combine(global Agent *a, global Agent *b) {
+    a->b_buffer = float4_add(a->b_buffer, b->b_buffer);
+    a->b_buffer_count += b->b_buffer_count;
+}
+
+void box_agent_see_clear(global BoxAgent *a) {
+    a->b_buffer = float4_null();
+    a->b_buffer_count = 0;
+}
+
+void box_agent_see_combine(global BoxAgent *a, global BoxAgent *b) {
+    a->b_buffer = float4_add(a->b_buffer, b->b_buffer);
+    a->b_buffer_count += b->b_buffer_count;
+}

@@ -75,3 +75,12 @@ void point_box_see(global Agent *a, global BoxAgent *b, constant SeeContext *c);
 void box_point_see(global BoxAgent *a, global Agent *b, constant SeeContext *c);
 void plain_see(global PlainAgent *a, global PlainAgent *b, constant PlainSeeContext *c);
 void plain_act(global PlainAgent *a, constant ActContext *c);
+/* For split see passes (tay_set_see_split): each see function above only
+ * adds to its seer's buffer and count, so a partial starts with them
+ * emptied (*_clear) and partials merge by adding them (*_combine, a += b). */
+void plain_see_clear(global PlainAgent *a);
+void plain_see_combine(global PlainAgent *a, global PlainAgent *b);
+void agent_see_clear(global Agent *a);
+void agent_see_combine(global Agent *a, global Agent *b);
+void box_agent_see_clear(global BoxAgent *a);
+void box_agent_see_combine(global BoxAgent *a, global BoxAgent *b);

@@ -62,7 +62,8 @@ class TayGpuPassShape(C.Structure):
                 ("seer_agent_size", C.c_uint), ("seen_agent_size", C.c_uint),
                 ("seer_is_point", C.c_int), ("seen_is_point", C.c_int),
                 ("seen_kind", C.c_int), ("dims", C.c_int), ("skip_self", C.c_int), ("context_size", C.c_uint),
-                ("see_order", C.c_int)]
+                ("see_order", C.c_int), ("split_lanes", C.c_uint), ("split_clear", C.c_char_p),
+                ("split_combine", C.c_char_p)]
 
 
 class TayGpuBatchPass(C.Structure):
@@ -116,6 +117,7 @@ _GPU = {
     "tay_set_lane_walk": (_i, [_vp, _u]),
     "tay_set_tile_seers": (_i, [_vp, _u]),
     "tay_set_see_order": (_i, [_vp, _i]),
+    "tay_set_see_split": (_i, [_vp, _u, _u, _vp, C.c_char_p, _vp, C.c_char_p]),
     "tay_set_order_bits": (_i, [_vp, _vp, _i]),
     "tay_set_fetch_mode": (None, [_vp, _i]),
     "tay_fetch_agents": (_i, [_vp, _vp]),

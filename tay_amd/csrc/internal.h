@@ -97,6 +97,7 @@ struct TayGpuProgram {
     struct Pass {
         TayGpuPassDesc desc;
         std::string func_name;
+        std::string split_clear, split_combine;     /* copies behind desc.split_clear / split_combine */
         hipFunction_t kernel;
         void *context;              /* device copy of the pass context */
         void *context_host;         /* pinned staging for the upload */

@@ -131,6 +131,9 @@ TayGpuPassShape shape_of(const TayGpuPassDesc &d) {
         sh.seen_kind = d.seen->kind;
         sh.skip_self = d.seer == d.seen && !d.self_see;
         sh.see_order = d.see_order;
+        sh.split_lanes = d.split_lanes > 1 ? d.split_lanes : 1u;
+        sh.split_clear = d.split_clear;
+        sh.split_combine = d.split_combine;
     }
     return sh;
 }
@@ -218,14 +221,23 @@ std::string compose(const char *const *sources, unsigned sources_count, const Ta
             append(text,
                    " void tay_pass_%u(TaySeeArgs args) {\n"
                    "    typedef tay_see_traits<decltype(%s)> T;\n"
-                   "    tay_see_tiles<T, %uu, %uu, %d, %s, %s, %d, %s, %uu, %s>(args,\n"
-                   "        [](T::seer *a, T::seen *b, T::context *c) { %s(a, b, c); });\n"
-                   "}\n",
+                   "    tay_see_tiles<T, %uu, %uu, %d, %s, %s, %d, %s, %uu, %s, %uu>(args,\n"
+                   "        [](T::seer *a, T::seen *b, T::context *c) { %s(a, b, c); },\n",
                    i, p.func_name, p.seer_agent_size / 4u, p.seen_agent_size / 4u,
                    narrow_kind(p.seer_is_point, p.seen_is_point), p.skip_self ? "true" : "false",
                    p.seen_kind == TAY_GPU_KIND_Z_GRID ? "true" : "false", p.dims, count_pairs ? "true" : "false",
-                   (p.context_size + 3u) / 4u, p.see_order ? "true" : "false",
-                   p.func_name);
+                   (p.context_size + 3u) / 4u, p.see_order && p.split_lanes <= 1 ? "true" : "false",
+                   p.split_lanes > 1 ? p.split_lanes : 1u, p.func_name);
+            if (p.split_lanes > 1)
+                append(text,
+                       "        [](T::seer *a) { %s(a); },\n"
+                       "        [](T::seer *a, T::seer *b) { %s(a, b); });\n"
+                       "}\n",
+                       p.split_clear, p.split_combine);
+            else
+                append(text,
+                       "        [](T::seer *) {}, [](T::seer *, T::seer *) {});\n"
+                       "}\n");
         } else {
             append(text,
                    "extern \"C\" __global__ __launch_bounds__(%u) void tay_pass_%u(TayActArgs args) {\n"
@@ -328,6 +340,24 @@ bool tay::valid_func_name(const char *name) {
     return valid_identifier(name);
 }
 
+namespace {
+/* A split see pass: 2, 4 or 8 lanes per seer and its two model functions
+ * named (0 or 1 lanes: no split). */
+bool valid_split(int type, unsigned lanes, const char *clear, const char *combine, unsigned i) {
+    if (type != TAY_GPU_PASS_SEE || lanes <= 1)
+        return true;
+    if (lanes != 2 && lanes != 4 && lanes != 8) {
+        set_error("pass %u: a split see pass has 2, 4 or 8 lanes per seer", i);
+        return false;
+    }
+    if (!valid_identifier(clear) || !valid_identifier(combine)) {
+        set_error("pass %u: split clear and combine function names are not C identifiers", i);
+        return false;
+    }
+    return true;
+}
+}  // namespace
+
 TAY_API int tay_gpu_compile_check(const char *const *sources, unsigned sources_count, const TayGpuPassShape *shapes,
                                   unsigned shapes_count, char *log, unsigned log_size, unsigned long long *code_size) {
     for (unsigned i = 0; i < shapes_count; ++i) {
@@ -336,6 +366,8 @@ TAY_API int tay_gpu_compile_check(const char *const *sources, unsigned sources_c
             set_error("pass %u: function name is not a C identifier", i);
             return 0;
         }
+        if (!valid_split(sh.type, sh.split_lanes, sh.split_clear, sh.split_combine, i))
+            return 0;
         if (sh.seer_agent_size % 4 || (sh.type == TAY_GPU_PASS_SEE && sh.seen_agent_size % 4) || sh.dims < 1 ||
             sh.dims > 3) {
             set_error("pass %u: agent sizes must be multiples of 4 and dims 1 to 3", i);
@@ -382,6 +414,8 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
             set_error("pass %u: function name is not a C identifier", i);
             return nullptr;
         }
+        if (!valid_split(d.type, d.split_lanes, d.split_clear, d.split_combine, i))
+            return nullptr;
         if (!d.seer || (d.type == TAY_GPU_PASS_SEE && !d.seen)) {
             set_error("pass %u: missing group", i);
             return nullptr;
@@ -458,6 +492,11 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
         pass.desc = passes[i];
         pass.func_name = passes[i].func_name;
         pass.desc.func_name = pass.func_name.c_str();
+        pass.desc.split_lanes = passes[i].split_lanes > 1 ? passes[i].split_lanes : 1u;
+        pass.split_clear = pass.desc.split_lanes > 1 ? passes[i].split_clear : "";
+        pass.split_combine = pass.desc.split_lanes > 1 ? passes[i].split_combine : "";
+        pass.desc.split_clear = pass.split_clear.c_str();
+        pass.desc.split_combine = pass.split_combine.c_str();
         pass.kernel = nullptr;
         pass.context = nullptr;
         pass.context_host = nullptr;
@@ -558,7 +597,9 @@ TAY_API int tay_gpu_program_run_pass(TayGpuProgram *p, unsigned pass_index) {
         for (int k = 0; k < 4; ++k)
             see.radii[k] = d.radii[k];
         see.pair_count = pass.pair_count;
-        see.tile_seers = p->tile_seers ? p->tile_seers : auto_tile_seers(a->count);
+        /* a split pass holds 64 / lanes seers per wavefront */
+        see.tile_seers = d.split_lanes > 1 ? 64u / d.split_lanes
+                                           : p->tile_seers ? p->tile_seers : auto_tile_seers(a->count);
         see.tiles = (a->count + see.tile_seers - 1u) / see.tile_seers;
         see.width = p->width < see.tile_seers ? p->width : see.tile_seers;
         see.lane_walk = p->lane_walk;

@@ -286,10 +286,13 @@ struct TaySeeView {
  * fixed order instead of the sweep order; TAY_BALANCE_RUN 64 (or M <= 64)
  * is the sweep order. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
-          bool COUNT, unsigned CTX_WORDS, bool ROW_WALK, bool BALANCED, class Call>
+          bool COUNT, unsigned CTX_WORDS, bool ROW_WALK, bool BALANCED, unsigned SPLIT, class Call, class Clear,
+          class Combine>
 __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i0, unsigned seers, unsigned width0,
                                                 unsigned lane_walk,
-                                                Call call) {
+                                                Call call, Clear clear, Combine combine) {
+    static_assert(SPLIT == 1u || SPLIT == 2u || SPLIT == 4u || SPLIT == 8u, "lanes per seer: 1, 2, 4 or 8");
+    static_assert(SPLIT == 1u || !BALANCED, "a split pass runs in the sweep order");
     static_assert(sizeof(typename T::seer) <= SEER_WORDS * 4u, "seer record larger than the group's agent size");
     static_assert(sizeof(typename T::seen) <= SEEN_WORDS * 4u, "seen record larger than the group's agent size");
     constexpr bool SEER_BOX = NARROW == TAY_NARROW_NP || NARROW == TAY_NARROW_NN;
@@ -303,8 +306,11 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
 
     const unsigned lane = threadIdx.x & 63u;
     const unsigned wave = threadIdx.x >> 6;
-    const unsigned i = i0 + lane;
-    const bool active = lane < seers && i < v.seer_end;
+    /* lane = seer slot x SPLIT + part: a split pass gives each seer SPLIT
+     * lanes, part 0 the seer itself, the others partials of it */
+    const unsigned sl = lane / SPLIT, part = lane % SPLIT;
+    const unsigned i = i0 + sl;
+    const bool active = sl < seers && i < v.seer_end;
     const unsigned ia = active ? i : i0;
 
     union {
@@ -315,6 +321,8 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
 #pragma unroll
     for (unsigned k = 0; k < SEER_WORDS; ++k)
         orig[k] = A.w[k] = v.seer_cols[k * v.seer_cap + ia];
+    if (SPLIT > 1u && part != 0u)
+        clear(&A.agent);    /* a partial starts with the sums emptied */
 
     /* position words: 2.. (min corner), 6.. (max corner of a box agent) */
     float a_lo[3] = {0.0f, 0.0f, 0.0f}, a_hi[3] = {0.0f, 0.0f, 0.0f}, r[3] = {0.0f, 0.0f, 0.0f};
@@ -372,6 +380,34 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         return bits;
     };
 
+    /* split pass: box-test this lane's slots part, part + SPLIT, ... of the
+     * staged chunk into bit t = slot / SPLIT, from the top bit down */
+    auto filter_split = [&]() {
+        const unsigned nt = (fill + SPLIT - 1u) / SPLIT;
+        unsigned bits = 0;
+        if (iso) {
+            for (unsigned t = nt; t-- > 0u;) {
+                const float *b = chunk_lo[wave][t * SPLIT + part];
+                float m = fabsf(a_lo[0] - b[0]);
+#pragma unroll
+                for (int k = 1; k < DIMS; ++k)
+                    m = fmaxf(m, fabsf(a_lo[k] - b[k]));
+                bits = tay_shift_in_ngt(bits, m, r0);
+            }
+        } else {
+            for (unsigned t = nt; t-- > 0u;) {
+                const unsigned c = t * SPLIT + part;
+                const float *b_lo = chunk_lo[wave][c];
+                const float *b_hi = SEEN_BOX ? chunk_hi[wave][c] : b_lo;
+                bits = bits + bits + (unsigned)tay_narrow<NARROW, DIMS>(a_lo, a_hi, b_lo, b_hi, r);
+            }
+        }
+        /* slot (nt - 1) * SPLIT + part lies past the chunk on some lanes */
+        if (nt != 0u && (nt - 1u) * SPLIT + part >= fill)
+            bits &= ~(1u << (nt - 1u));
+        return bits;
+    };
+
     auto process = [&]() {
         if (lane < fill) {
             const unsigned off = cand * 4u;
@@ -391,18 +427,6 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
         }
         tay_wave_sync();
 
-        const unsigned lo_bits = filter(0u, fill < 32u ? fill : 32u);
-        const unsigned hi_bits = fill > 32u ? filter(32u, fill) : 0u;
-        unsigned long long mask = ((unsigned long long)hi_bits << 32) | lo_bits;
-        if (SKIP_SELF && self_slot < 64u)
-            mask &= ~(1ull << self_slot);
-        self_slot = 64u;
-        if (BALANCED)
-            mask &= chunk_valid;
-        mask = member ? mask : 0ull;
-        if (COUNT)
-            pairs += (unsigned)__popcll(mask);
-
         auto visit = [&](unsigned c) {
             const float4 rec = *(const float4 *)chunk_lo[wave][c];
             union {
@@ -419,6 +443,39 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                 B.w[2 + k] = __float_as_uint(pos[k]);
             call(&A.agent, &B.agent, ctx);
         };
+        if constexpr (SPLIT > 1u) {
+            /* this lane's slots part, part + SPLIT, ...: bit t is slot
+             * t * SPLIT + part, taken in ascending order */
+            unsigned m = filter_split();
+            if (SKIP_SELF && self_slot < 64u && self_slot % SPLIT == part)
+                m &= ~(1u << (self_slot / SPLIT));
+            self_slot = 64u;
+            m = member ? m : 0u;
+            if (COUNT)
+                pairs += (unsigned)__popc(m);
+            const unsigned rounds = tay_wave_max_u7((unsigned)__popc(m));
+            for (unsigned round = 0; round < rounds; ++round) {
+                const bool has = m != 0u;
+                const unsigned c = ((unsigned)__builtin_ctz(m) & 31u) * SPLIT + part;
+                m &= m - 1u;
+                if (has)
+                    visit(c);
+            }
+            tay_wave_sync();
+            fill = 0;
+            return;
+        }
+        const unsigned lo_bits = filter(0u, fill < 32u ? fill : 32u);
+        const unsigned hi_bits = fill > 32u ? filter(32u, fill) : 0u;
+        unsigned long long mask = ((unsigned long long)hi_bits << 32) | lo_bits;
+        if (SKIP_SELF && self_slot < 64u)
+            mask &= ~(1ull << self_slot);
+        self_slot = 64u;
+        if (BALANCED)
+            mask &= chunk_valid;
+        mask = member ? mask : 0ull;
+        if (COUNT)
+            pairs += (unsigned)__popcll(mask);
         /* The lowest set bit and its removal are computed on every lane (an
          * empty mask stays empty), so the mask needs no merge after the
          * body.  A 32-bit mask costs one find-first-bit and two operations
@@ -502,7 +559,7 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
             __builtin_amdgcn_readfirstlane(v.seer_end - i0 < seers ? v.seer_end - i0 : seers);
         unsigned beg = 0, width = width0;
         while (beg < n_active) {
-            member = active && lane - beg < width;
+            member = active && sl - beg < width;
             int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
             unsigned cells = 1;
 #pragma unroll
@@ -537,7 +594,11 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                  * cells, in the same order: fewer waits per seer, for about
                  * forty more registers, which the batched kernel has to spare
                  * and the plain kernels, compiled for seven or eight
-                 * wavefronts per SIMD, would spill. */
+                 * wavefronts per SIMD, would spill.  In a split pass the
+                 * seer's lanes share its cell rows: lane `part` walks the
+                 * rows (x, y) whose index in the seer's box is part mod
+                 * SPLIT, into its partial. */
+                static_assert(!ROW_WALK || SPLIT == 1u, "the row walk of batched tiles is not split");
                 if (member) {
                     constexpr unsigned WALK = 2, ZB = 4, RW = 4;
                     int plo[3] = {0, 0, 0}, phi[3] = {0, 0, 0};
@@ -620,9 +681,13 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
                                     }
                                 }
                     } else {
+                        const unsigned ny = (unsigned)(phi[1] - plo[1] + 1);
                         for (int x = plo[0]; x <= phi[0]; ++x)
                             for (int y = plo[1]; y <= phi[1]; ++y)
                                 for (int z0 = plo[2]; z0 <= phi[2]; z0 += (int)WALK) {
+                                    if (SPLIT > 1u &&
+                                        ((unsigned)(x - plo[0]) * ny + (unsigned)(y - plo[1])) % SPLIT != part)
+                                        break;
                                     TayCellRange crs[WALK];
 #pragma unroll
                                     for (unsigned q = 0; q < WALK; ++q) {
@@ -801,7 +866,23 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
      * are not stored; every word it may write is stored, changed or not
      * (the storage is updated in place), so no copy of the loaded words
      * has to stay in registers through the sweep. */
-    if (active) {
+    if constexpr (SPLIT > 1u) {
+        /* merge the partials into part 0 in a fixed tree: (0 + 1), (2 + 3),
+         * ..., then pairs of those */
+#pragma unroll
+        for (unsigned step = 1; step < SPLIT; step <<= 1) {
+            union {
+                typename T::seer agent;
+                unsigned w[SEER_WORDS];
+            } B;
+#pragma unroll
+            for (unsigned k = 0; k < SEER_WORDS; ++k)
+                B.w[k] = __shfl_xor(A.w[k], (int)step);
+            if ((part & (2u * step - 1u)) == 0u)
+                combine(&A.agent, &B.agent);
+        }
+    }
+    if (active && part == 0u) {
 #pragma unroll
         for (unsigned k = 0; k < SEER_WORDS; ++k) {
             const bool unchanged = A.w[k] == orig[k];
@@ -820,8 +901,8 @@ __device__ __forceinline__ void tay_add_pairs(unsigned long long *counter, unsig
 
 /* Plain see kernel: wavefront k of the grid owns tile k. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
-          bool COUNT, unsigned CTX_WORDS, bool BALANCED, class Call>
-__device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call) {
+          bool COUNT, unsigned CTX_WORDS, bool BALANCED, unsigned SPLIT, class Call, class Clear, class Combine>
+__device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call, Clear clear, Combine combine) {
     constexpr unsigned WAVES = TAY_SEE_BLOCK / 64u;
     const unsigned tile = tay_xcd_block() * WAVES + (threadIdx.x >> 6);
     if (tile >= args.tiles)
@@ -842,9 +923,9 @@ __device__ __forceinline__ void tay_see_tiles(const TaySeeArgs &args, Call call)
     for (int k = 0; k < 4; ++k)
         v.radii[k] = args.radii[k];
     const unsigned pairs = tay_see_tile<T, SEER_WORDS, SEEN_WORDS, NARROW, SKIP_SELF, SEEN_GRID, DIMS, COUNT,
-                                        CTX_WORDS, false, BALANCED>(v, tile * args.tile_seers, args.tile_seers,
-                                                                  args.width,
-                                                   args.lane_walk, call);
+                                        CTX_WORDS, false, BALANCED, SPLIT>(v, tile * args.tile_seers,
+                                                                         args.tile_seers, args.width,
+                                                                         args.lane_walk, call, clear, combine);
     if (COUNT)
         tay_add_pairs(args.pair_count, pairs);
 }
@@ -887,8 +968,9 @@ __device__ __forceinline__ void tay_see_batch(const TayBatchSeeArgs &args, Call 
 #pragma unroll
         for (int d = 0; d < 4; ++d)
             v.radii[d] = tay_uniform(args.radii[4u * b + d]);
-        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS, true, false>(
-            v, tile * 64u, 64u, args.width, args.lane_walk, call);
+        pairs += tay_see_tile<T, WORDS, WORDS, TAY_NARROW_PP, SKIP_SELF, true, DIMS, COUNT, CTX_WORDS, true, false, 1u>(
+            v, tile * 64u, 64u, args.width, args.lane_walk, call, [](typename T::seer *) {},
+            [](typename T::seer *, typename T::seer *) {});
         unsigned q = 0;
         if ((threadIdx.x & 63u) == 0u)
             q = atomicAdd(args.queue, 1u);

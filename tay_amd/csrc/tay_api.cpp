@@ -62,6 +62,9 @@ struct TayPass {
     void *context;
     unsigned context_size;
     int has_func;
+    unsigned split_lanes;       /* see passes: lanes per seer (tay_set_see_split), 0 or 1: no split */
+    char split_clear[kMaxFuncName];
+    char split_combine[kMaxFuncName];
 };
 
 struct TayState {
@@ -263,7 +266,20 @@ int state_compile(TayState *state) {
         desc[i].radii[3] = p.radii.w;
         desc[i].self_see = p.self_see;
         desc[i].context_size = p.context_size;
-        desc[i].see_order = state->see_order;
+        /* the balanced order and split passes apply where the seen group is
+         * a GPU Z-grid and the seer group a GPU space of its own (not a CPU
+         * Z-grid moved to the device), as the oracle states them */
+        const bool tiled = p.type == TAY_GPU_PASS_SEE && p.seen->type == TAY_GPU_Z_GRID &&
+                           p.seer->type != TAY_CPU_Z_GRID;
+        desc[i].see_order = tiled ? state->see_order : TAY_SEE_ORDER_REFERENCE;
+        if (tiled && p.split_lanes > 1) {
+            desc[i].split_lanes = p.split_lanes;
+            desc[i].split_clear = p.split_clear;
+            desc[i].split_combine = p.split_combine;
+            desc[i].see_order = TAY_SEE_ORDER_REFERENCE;
+        } else {
+            desc[i].split_lanes = 1;
+        }
     }
     static thread_local char log[1 << 16];
     log[0] = '\0';
@@ -680,6 +696,30 @@ TAY_API int tay_set_see_order(TayState *state, int order) {
         state->see_order = order;
         state->recompile = 1;
     }
+    return 1;
+}
+
+TAY_API int tay_set_see_split(TayState *state, unsigned pass_index, unsigned lanes, void (*clear)(void *),
+                              const char *clear_name, void (*combine)(void *, void *), const char *combine_name) {
+    (void)clear;
+    (void)combine;
+    if (pass_index >= state->passes_count || state->passes[pass_index].type != TAY_GPU_PASS_SEE) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see split: no see pass at that index");
+        return 0;
+    }
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see split: lanes per seer must be 1, 2, 4 or 8");
+        return 0;
+    }
+    TayPass &p = state->passes[pass_index];
+    if (lanes > 1 && (!clear_name || !combine_name || !clear_name[0] || !combine_name[0])) {
+        set_error(state, TAY_ERROR_NOT_IMPLEMENTED, "see split: clear and combine functions must be named");
+        return 0;
+    }
+    p.split_lanes = lanes;
+    std::snprintf(p.split_clear, kMaxFuncName, "%s", lanes > 1 ? clear_name : "");
+    std::snprintf(p.split_combine, kMaxFuncName, "%s", lanes > 1 ? combine_name : "");
+    state->recompile = 1;
     return 1;
 }
 

@@ -234,6 +234,20 @@ def sph_model(ctx, h=1.0):
                  part=h)
 
 
+# Clear and combine functions of the see functions that only add to their
+# seer (tay_amd/agent_src), for split see passes.
+SPLIT_FUNCS = {
+    "flock_see": ("flock_see_clear", "flock_see_combine"),
+    "plain_see": ("plain_see_clear", "plain_see_combine"),
+    "agent_see": ("agent_see_clear", "agent_see_combine"),
+    "box_agent_see": ("box_agent_see_clear", "box_agent_see_combine"),
+    "point_box_see": ("agent_see_clear", "agent_see_combine"),
+    "box_point_see": ("box_agent_see_clear", "box_agent_see_combine"),
+    "sph_density": ("sph_density_clear", "sph_density_combine"),
+    "sph_forces": ("sph_forces_clear", "sph_forces_combine"),
+}
+
+
 class Simulation:
     """One Tay state with one group running `model` on `lib`.
 
@@ -244,6 +258,7 @@ class Simulation:
                  sources=None, part=None, box=None):
         self.lib = lib
         self.model = model
+        self.funcs = funcs
         self.state = lib.tay_create_state()
         if not self.state:
             raise RuntimeError("tay_create_state failed")
@@ -279,6 +294,16 @@ class Simulation:
             raise RuntimeError("tay_run failed: error %d %s" % (self.lib.tay_get_error(self.state),
                                                                  self.lib.error_message(self.state)))
         return done
+
+    def see_split(self, pass_index, lanes):
+        """Splits see pass `pass_index` over `lanes` lanes per seer
+        (tay_set_see_split) with the see function's clear and combine
+        functions (SPLIT_FUNCS)."""
+        names = SPLIT_FUNCS[self.model.passes[pass_index].func]
+        fns = [C.cast(getattr(self.funcs, f), C.c_void_p) if self.funcs is not None else None for f in names]
+        if not self.lib.tay_set_see_split(self.state, pass_index, lanes, fns[0], names[0].encode(), fns[1],
+                                          names[1].encode()):
+            raise RuntimeError("tay_set_see_split failed: %s" % self.lib.error_message(self.state))
 
     def agents(self):
         return self.lib.get_agents(self.state, self.group, self.count, self.model.dtype)

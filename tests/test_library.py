@@ -68,6 +68,34 @@ def test_models_compile_for_gfx950_without_gpu(tay):
     assert size > 1000
 
 
+def test_split_passes_compile_without_gpu(tay):
+    """Split see passes (2, 4 and 8 lanes per seer) of every model with clear
+    and combine functions, over 1 to 3 dimensions and point and box groups;
+    a split pass naming a function that is not a C identifier is refused."""
+    def split(func, dtype, lanes, seen=None, dims=3, point=1, seen_point=None):
+        clear, combine = W.SPLIT_FUNCS[func]
+        seen = seen or dtype
+        sp = point if seen_point is None else seen_point
+        return api.TayGpuPassShape(0, func.encode(), dtype.itemsize, seen.itemsize, point, sp,
+                                   api.TAY_GPU_KIND_Z_GRID, dims, 1 if seen is dtype else 0, 0, 0, lanes,
+                                   clear.encode(), combine.encode())
+    shapes = []
+    for lanes in (2, 4, 8):
+        shapes += [split("flock_see", A.FLOCK_AGENT, lanes), split("plain_see", A.PLAIN_AGENT, lanes),
+                   split("box_agent_see", A.BOX_AGENT, lanes, point=0),
+                   split("point_box_see", A.AGENT, lanes, seen=A.BOX_AGENT, seen_point=0),
+                   split("box_point_see", A.BOX_AGENT, lanes, seen=A.AGENT, point=0, seen_point=1)]
+        shapes += [split("agent_see", A.AGENT, lanes, dims=d) for d in (1, 2, 3)]
+        shapes += [split("sph_density", A.SPH_PARTICLE, lanes), split("sph_forces", A.SPH_PARTICLE, lanes)]
+    ok, log, size = compile_check(tay, A.source_texts(), shapes)
+    assert ok, log
+    assert size > 1000
+    bad = split("plain_see", A.PLAIN_AGENT, 2)
+    bad.split_combine = b"not a name"
+    ok, log, _ = compile_check(tay, A.source_texts(), [bad])
+    assert not ok
+
+
 def test_batched_programs_compile_without_gpu(tay):
     P = api.TayGpuBatchPass
     srcs = [s.encode() for s in A.source_texts()]
