@@ -88,8 +88,9 @@ B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
 # profiles/see_flock1M_pmc.md).  A counter run cannot be made from inside
 # this process: the figure is carried here, with the commit of the kernel it
 # was measured on, and re-measured whenever the see kernel changes.
-SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0, "kernel": "see kernel of commit c7e55b7",
-               "bytes": int((43705.6 * 2.000 + 54675.3 * 1.000) * 1024)}
+SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0,
+               "reference": {"kernel": "see kernel of commit c7e55b7",
+                             "bytes": int((43705.6 * 2.000 + 54675.3 * 1.000) * 1024)}}
 # The SPH line's see passes on 2M particles (profiles/see_sph2M_pmc.md),
 # same calibration.  Algorithmic bytes per particle: forces, the seer reads
 # p, v, pressure and both accumulators (15 words) and writes the
@@ -291,22 +292,27 @@ def bench_flocking(lib, args):
     recs = W.flock_agents(args.agents, seed=1, clusters=64, sigma=30.0, box=2000.0)
     model = W.flock_model(radius=25.0, weights=(1.5, 1.0, 1.0), dt=0.1, max_speed=2.0)
 
-    def window(advance_steps, warmup, steps, name):
+    def setup(s, split):
+        lib.tay_set_fetch_mode(s.state, api.TAY_FETCH_ON_DEMAND)
+        if args.order_bits >= 0:
+            lib.tay_set_order_bits(s.state, s.group, args.order_bits)
+        if split:
+            s.see_split(0, args.flock_split)
+            lib.tay_set_lane_walk(s.state, args.flock_lane_walk)
+
+    def window(advance_steps, warmup, steps, name, split=True):
+        split = split and args.flock_split > 1
         # counting twin: same inputs, same trajectory, pair counts of the timed steps
         with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as twin:
-            lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
+            setup(twin, split)
             lib.tay_set_pair_counting(twin.state, 1)
-            if args.order_bits >= 0:
-                lib.tay_set_order_bits(twin.state, twin.group, args.order_bits)
             advance(twin, advance_steps, name + " counting twin")
             if warmup:
                 twin.run(warmup)
             twin.run(steps)
             p_see = lib.pair_counts(twin.state, 0)[1]
         with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID, part=args.cell_size) as sim:
-            lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
-            if args.order_bits >= 0:
-                lib.tay_set_order_bits(sim.state, sim.group, args.order_bits)
+            setup(sim, split)
             advance(sim, advance_steps, name + " timed state")
             if warmup:
                 sim.run(warmup)
@@ -326,8 +332,9 @@ def bench_flocking(lib, args):
     see_avg_s = see_ms / 1e3 / see_launches
     flop_per_launch = p_see / args.steps * f_pair
     hbm = B_SEE_PER_AGENT * args.agents / see_avg_s / 1e9
-    traffic = SEE_TRAFFIC["bytes"] if (args.agents, args.cell_size, args.advance) == (
-        SEE_TRAFFIC["agents"], SEE_TRAFFIC["cell_size"], 0) else None
+    setting = "split%d_walk%d" % (args.flock_split, args.flock_lane_walk) if args.flock_split > 1 else "reference"
+    traffic = SEE_TRAFFIC[setting]["bytes"] if (args.agents, args.cell_size, args.advance) == (
+        SEE_TRAFFIC["agents"], SEE_TRAFFIC["cell_size"], 0) and setting in SEE_TRAFFIC else None
 
     out = {
         "metric": METRIC,
@@ -345,6 +352,8 @@ def bench_flocking(lib, args):
         "config": {"workload": "flocking-1M", "agents": args.agents, "clusters": 64, "sigma": 30.0,
                    "box": 2000.0, "radius": 25.0, "cell_size": args.cell_size, "weights": [1.5, 1.0, 1.0], "dt": 0.1,
                    "max_speed": 2.0, "structure": "TAY_GPU_Z_GRID",
+                   "see_split": {"lanes": args.flock_split, "lane_walk": args.flock_lane_walk}
+                   if args.flock_split > 1 else None,
                    **({"order_bits": args.order_bits} if args.order_bits >= 0 else {}),
                    **({"advanced_steps": args.advance} if args.advance else {})},
         "steps_per_s": args.steps / wall,
@@ -366,6 +375,16 @@ def bench_flocking(lib, args):
         "roofline_hbm": {"achieved": hbm, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": hbm / PEAK_HBM_GBS,
                          "bytes_per_agent": B_SEE_PER_AGENT},
     }
+    if args.flock_split > 1 and not args.no_reference_order:
+        # the unsplit pass (reference sweep order, default lane walk) on the same window
+        rp, rw, ((rsee, rl), _), rfetch, _, _ = window(args.advance, args.warmup, args.steps, "unsplit", split=False)
+        out["reference_see_order"] = {"pairs_per_s": rp / (rsee / 1e3), "steps_per_s": args.steps / rw,
+                                      "steps_per_s_resident": args.steps / (rw - rfetch),
+                                      "ms_per_step": rw * 1e3 / args.steps, "see_ms_per_step": rsee / rl,
+                                      "p_see_per_step": rp / args.steps,
+                                      "note": "the see pass unsplit, each seer's pairs in the reference CPU "
+                                              "grids' sweep order; same pairs and counts"}
+        progress("flocking: unsplit window done")
     if args.steps < CONFIGURED_STEPS and not args.advance and not args.no_full_run:
         fp, fw, ((fsee, fl), _), ffetch, _, _ = window(0, 0, CONFIGURED_STEPS, "full run")
         out["full_run"] = {"steps": CONFIGURED_STEPS, "pairs_per_s": fp / (fsee / 1e3),
@@ -388,32 +407,50 @@ def bench_c1(lib, args):
     n, steps, radius = 10_000, 100, 40.0
     recs = W.plain_agents(n, seed=1)
     model = W.plain_model(radius)
-    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as twin:
-        lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
-        lib.tay_set_pair_counting(twin.state, 1)
-        twin.run(2)
-        twin.run(steps)
-        p_see = lib.pair_counts(twin.state, 0)[1]
-    # a step of this state takes about 0.1 ms, so the events that time the
-    # passes (a few microseconds each) would show in its wall time: steps/s
-    # comes from a run without them, the pass times from a second run of the
-    # same steps
-    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
-        sim.run(2)
-        wall, _, fetch = timed_run(lib, sim, steps, pass_timing=False)
-    with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
-        sim.run(2)
-        _, ((see_ms, launches), _), _ = timed_run(lib, sim, steps)
-        p_in = int(sim.agents()["last_ncount"].sum())
+
+    def state(split):
+        sim = W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID)
+        if split > 1:
+            sim.see_split(0, split)
+        return sim
+
+    def runs(split):
+        with state(split) as twin:
+            lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
+            lib.tay_set_pair_counting(twin.state, 1)
+            twin.run(2)
+            twin.run(steps)
+            p_see = lib.pair_counts(twin.state, 0)[1]
+        # a step of this state takes about 0.1 ms, so the events that time
+        # the passes (a few microseconds each) would show in its wall time:
+        # steps/s comes from a run without them, the pass times from a
+        # second run of the same steps
+        with state(split) as sim:
+            sim.run(2)
+            wall, _, fetch = timed_run(lib, sim, steps, pass_timing=False)
+        with state(split) as sim:
+            sim.run(2)
+            _, ((see_ms, launches), _), _ = timed_run(lib, sim, steps)
+            p_in = int(sim.agents()["last_ncount"].sum())
+        return p_see, wall, fetch, see_ms, launches, p_in
+
+    p_see, wall, fetch, see_ms, launches, p_in = runs(args.c1_split)
     shares = {"inside": p_in / (p_see / steps)}
     f_pair = f_pair_of(PLAIN_F, shares)
     out = {"workload": "plain-10k", "agents": n, "radius": radius, "cell_size": 2 * radius, "steps": steps,
-           "p_see_per_step": p_see / steps, "pairs_per_s": p_see / (see_ms / 1e3), "steps_per_s": steps / wall,
+           "see_split": args.c1_split if args.c1_split > 1 else None, "p_see_per_step": p_see / steps,
+           "pairs_per_s": p_see / (see_ms / 1e3), "steps_per_s": steps / wall,
            "steps_per_s_resident": steps / (wall - fetch), "ms_per_step": wall * 1e3 / steps,
            "see_ms_per_step": see_ms / launches,
            "roofline": roofline(p_see / steps * f_pair, see_ms / 1e3 / launches, "see (tay_pass_0, plain_see)",
                                 f_pair, "launch-latency bound: the whole state is 0.8 MB; F = P_see x f_pair, "
                                         "share inside r from the last step's counts", branch_shares=shares)}
+    if args.c1_split > 1 and not args.no_reference_order:
+        rp, rwall, rfetch, rsee, rl, _ = runs(1)
+        out["reference_see_order"] = {"pairs_per_s": rp / (rsee / 1e3), "steps_per_s": steps / rwall,
+                                      "steps_per_s_resident": steps / (rwall - rfetch),
+                                      "ms_per_step": rwall * 1e3 / steps, "see_ms_per_step": rsee / rl,
+                                      "note": "the see pass unsplit (reference sweep order); same pairs"}
     if not args.no_cpu_baseline:
         base = {}
         for name, space, k in (("CpuSimple", api.TAY_CPU_SIMPLE, 10), ("CpuGrid", api.TAY_CPU_GRID, steps),
@@ -575,9 +612,18 @@ def bench_sph(lib, args):
     model = W.sph_model(ctx, 1.0)
     steps, warm = args.sph_steps, 2
     order = SEE_ORDERS[args.sph_see_order]
+    dsplit = args.sph_density_split
+
+    def split(sim):
+        # the density pass split over dsplit lanes (it then runs in the
+        # sweep order); the force pass in the state's see order
+        if dsplit > 1:
+            sim.see_split(0, dsplit)
+
     with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as twin:
         lib.tay_set_fetch_mode(twin.state, api.TAY_FETCH_ON_DEMAND)
         lib.tay_set_see_order(twin.state, order)
+        split(twin)
         lib.tay_set_pair_counting(twin.state, 1)
         twin.run(warm)
         twin.run(steps)
@@ -587,6 +633,8 @@ def bench_sph(lib, args):
         with W.Simulation(lib, model, recs, api.TAY_GPU_Z_GRID) as sim:
             lib.tay_set_fetch_mode(sim.state, api.TAY_FETCH_ON_DEMAND)
             lib.tay_set_see_order(sim.state, SEE_ORDERS[name])
+            if name == args.sph_see_order:
+                split(sim)
             sim.run(warm)
             lib.tay_fetch_agents(sim.state, None)
             first = sim.agents()
@@ -606,11 +654,14 @@ def bench_sph(lib, args):
     share_d = 0.5 * (inside[0][0] + inside[1][0])
     share_f = 0.5 * (inside[0][1] + inside[1][1])
     measured = len(recs) == SPH_TRAFFIC["particles"]
-    traffic = SPH_TRAFFIC[args.sph_see_order]
+    traffic = dict(SPH_TRAFFIC[args.sph_see_order])
+    if dsplit > 1:
+        traffic["density"] = SPH_TRAFFIC.get("density_split%d" % dsplit)
     f_d = f_pair_of(SPH_DENSITY_F, {"inside": share_d})
     f_f = f_pair_of(SPH_FORCES_F, {"inside": share_f})
     out = {"workload": "SPH dam break, %d particles" % len(recs), "particles": len(recs), "h": 1.0, "dt": 1e-3,
-           "steps": steps, "see_order": args.sph_see_order, "p_see_per_step": [p / steps for p in pairs],
+           "steps": steps, "see_order": args.sph_see_order,
+           "density_split": dsplit if dsplit > 1 else None, "p_see_per_step": [p / steps for p in pairs],
            "pairs_per_s": sum(pairs) / (sum(see) / 1e3), "steps_per_s": steps / wall,
            "steps_per_s_resident": steps / (wall - fetch), "ms_per_step": wall * 1e3 / steps,
            "see_ms_per_step": [t / steps for t in see], "act_ms_per_step": [t / steps for t in act],
@@ -623,10 +674,12 @@ def bench_sph(lib, args):
                                 branch_shares={"inside": share_f},
                                 density=roofline(pairs[0] / steps * f_d, see[0] / 1e3 / steps,
                                                  "see (tay_pass_0, sph_density)", f_d, "",
-                                                 traffic=traffic["density"] if measured else None,
+                                                 traffic=traffic["density"] if measured and traffic["density"]
+                                                 else None,
                                                  traffic_bytes_algorithmic=B_SPH_DENSITY_PER_PARTICLE * len(recs),
                                                  branch_shares={"inside": share_d})),
-           other + "_see_order": {"steps_per_s": steps / o_wall, "ms_per_step": o_wall * 1e3 / steps,
+           other + "_see_order": {"note": "both see passes unsplit in this order",
+                                  "steps_per_s": steps / o_wall, "ms_per_step": o_wall * 1e3 / steps,
                                   "see_ms_per_step": [o_times[i][0] / steps for i in (0, 2)],
                                   "pairs_per_s": sum(pairs) / (sum(o_times[i][0] for i in (0, 2)) / 1e3)}}
     if not args.no_cpu_baseline:
@@ -767,6 +820,16 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 10k plain-agent measurement")
     ap.add_argument("--sph-particles", type=int, default=2_000_000)
     ap.add_argument("--sph-steps", type=int, default=500)
+    ap.add_argument("--flock-split", type=int, choices=(1, 2, 4, 8), default=4,
+                    help="lanes per seer of the flocking see pass (tay_set_see_split; 1: unsplit)")
+    ap.add_argument("--flock-lane-walk", type=int, default=0,
+                    help="tay_set_lane_walk for the split flocking pass (0: lane groups halve, never walk)")
+    ap.add_argument("--no-reference-order", action="store_true",
+                    help="skip the unsplit flocking and 10k plain runs timed beside the split ones")
+    ap.add_argument("--c1-split", type=int, choices=(1, 2, 4, 8), default=8,
+                    help="lanes per seer of the 10k plain see pass (1: unsplit)")
+    ap.add_argument("--sph-density-split", type=int, choices=(1, 2, 4, 8), default=4,
+                    help="lanes per seer of the SPH density pass of the timed state (1: unsplit)")
     ap.add_argument("--sph-see-order", choices=sorted(SEE_ORDERS), default="balanced",
                     help="see order of the timed SPH run (the other order is timed too, for comparison)")
     ap.add_argument("--no-sph", action="store_true", help="skip the SPH measurement")

@@ -167,7 +167,11 @@ std::string compose_head(const char *const *sources, unsigned sources_count) {
  * 1M-agent see pass moves 125 MB instead of 207 MB at equal time (first
  * steps 4.92 ms at 7, 4.97 at 8; 5.26 at 6).  The persistent batch kernel
  * keeps the compiler's choice: its sparse instances walk per lane, scattered
- * loads that more resident wavefronts only make miss more.
+ * loads that more resident wavefronts only make miss more.  A split see
+ * pass stays at 8 whatever it spills: its spills sit around the lane-group
+ * loop, not in the filter and compact loops, and the 1M flocking pass split
+ * over 4 lanes runs 3.21 ms per step at 8 (128 bytes of scratch), 3.36 at 7
+ * (104 bytes), 3.51 at 6, 3.80 at 5.
  * TAY_SEE_WAVES_PER_EU=<n> (1-8) fixes the budget of every see kernel, 0
  * leaves it to the compiler. */
 constexpr int kSeeWavesFirst = 8;
@@ -453,11 +457,11 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
         return nullptr;
     }
     if (forced < 0) {
-        /* see kernels that keep values in scratch at the first budget are
-         * compiled again for the second */
+        /* unsplit see kernels that keep values in scratch at the first
+         * budget are compiled again for the second */
         bool again = false;
         for (unsigned i = 0; i < passes_count; ++i) {
-            if (passes[i].type != TAY_GPU_PASS_SEE)
+            if (passes[i].type != TAY_GPU_PASS_SEE || passes[i].split_lanes > 1)
                 continue;
             char name[64];
             std::snprintf(name, sizeof(name), "tay_pass_%u", i);
