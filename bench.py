@@ -90,7 +90,10 @@ B_SEE_PER_AGENT = 4 * (17 + 11 + 6)
 # was measured on, and re-measured whenever the see kernel changes.
 SEE_TRAFFIC = {"agents": 1_000_000, "cell_size": 25.0,
                "reference": {"kernel": "see kernel of commit c7e55b7",
-                             "bytes": int((43705.6 * 2.000 + 54675.3 * 1.000) * 1024)}}
+                             "bytes": int((43705.6 * 2.000 + 54675.3 * 1.000) * 1024)},
+               # split over 4 lanes, lane walk 0 (profiles/see_flock1M_split_pmc.md)
+               "split4_walk0": {"kernel": "split see kernel of commit be0ef4a, 8 wavefronts per SIMD",
+                                "bytes": int((147273.2 * 2.000 + 404267.9 * 1.000) * 1024)}}
 # The SPH line's see passes on 2M particles (profiles/see_sph2M_pmc.md),
 # same calibration.  Algorithmic bytes per particle: forces, the seer reads
 # p, v, pressure and both accumulators (15 words) and writes the
@@ -103,7 +106,9 @@ SPH_TRAFFIC = {"particles": 2_000_000,
                              "density": int((19950.6 * 2.000 + 7812.5 * 1.000) * 1024)},
                "balanced": {"kernel": "see kernels of commit aaf7979",
                             "forces": int((128419.3 * 2.000 + 124785.2 * 1.000) * 1024),
-                            "density": int((19950.9 * 2.000 + 7812.5 * 1.000) * 1024)}}
+                            "density": int((19950.9 * 2.000 + 7812.5 * 1.000) * 1024)},
+               # the density pass split over 4 lanes (profiles/see_flock1M_split_pmc.md)
+               "density_split4": int((19968.6 * 2.000 + 7812.7 * 1.000) * 1024)}
 B_SPH_FORCES_PER_PARTICLE = 4 * (15 + 8 + 8)
 B_SPH_DENSITY_PER_PARTICLE = 4 * (4 + 1 + 3)
 
@@ -368,7 +373,8 @@ def bench_flocking(lib, args):
             flop_per_launch, see_avg_s, "see (tay_pass_0, flock_see)", f_pair,
             "FP32 vector-issue bound, no matrix instructions; F = P_see x f_pair, f_pair = %d (box test) + %d "
             "(every pair) + %d x inside + %d x front + %d x separation, shares measured on the window's first "
-            "and last agents; traffic from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/see_flock1M_pmc.md)"
+            "and last agents; traffic from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/see_flock1M_split_pmc.md "
+            "for the split pass, profiles/see_flock1M_pmc.md unsplit)"
             % (F_BOX_TEST, FLOCK_F["all"], FLOCK_F["inside"], FLOCK_F["front"], FLOCK_F["separation"]),
             traffic=traffic, branch_shares=shares, branch_shares_first=shares_first,
             branch_shares_last=shares_last, traffic_bytes_algorithmic=B_SEE_PER_AGENT * args.agents),

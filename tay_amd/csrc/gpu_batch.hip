@@ -751,8 +751,8 @@ TAY_API TayGpuBatch *tay_gpu_batch_create(const char *const *sources, unsigned s
             int per_cu = 0;
             ok = device_alloc((void **)&p.radii, 16ull * instances, "batch radii") &&
                  hip_ok(hipMemsetAsync(p.radii, 0, 16ull * instances, s), "hipMemsetAsync") &&
-                 device_alloc((void **)&p.pair_count, 8, "pair counter") &&
-                 hip_ok(hipMemsetAsync(p.pair_count, 0, 8, s), "hipMemsetAsync") &&
+                 device_alloc((void **)&p.pair_count, 8ull * TAY_PAIR_SLOTS, "pair counters") &&
+                 hip_ok(hipMemsetAsync(p.pair_count, 0, 8ull * TAY_PAIR_SLOTS, s), "hipMemsetAsync") &&
                  hip_ok(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.kernel, kSeeBlock, 0),
                         "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
             /* tuning: TAY_BATCH_SEE_BLOCKS_PER_CU=<n> replaces the occupancy
@@ -1011,10 +1011,7 @@ TAY_API int tay_gpu_batch_pair_count(TayGpuBatch *b, unsigned pass_index, unsign
         set_error("batch was created without pair counting");
         return 0;
     }
-    hipStream_t s = stream();
-    return s && hip_ok(hipMemcpyAsync(pairs, b->passes[pass_index].pair_count, 8, hipMemcpyDeviceToHost, s),
-                       "read pair count") &&
-           hip_ok(hipStreamSynchronize(s), "read pair count");
+    return read_pair_counters(b->passes[pass_index].pair_count, pairs);
 }
 
 TAY_API int tay_gpu_batch_reset_pair_counts(TayGpuBatch *b) {
@@ -1022,7 +1019,7 @@ TAY_API int tay_gpu_batch_reset_pair_counts(TayGpuBatch *b) {
     if (!s)
         return 0;
     for (auto &p : b->passes)
-        if (p.pair_count && !hip_ok(hipMemsetAsync(p.pair_count, 0, 8, s), "reset pair count"))
+        if (p.pair_count && !hip_ok(hipMemsetAsync(p.pair_count, 0, 8ull * TAY_PAIR_SLOTS, s), "reset pair count"))
             return 0;
     return 1;
 }

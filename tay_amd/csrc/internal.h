@@ -26,6 +26,8 @@ hipStream_t stream();
 
 /* Device allocation helpers; false with the error recorded on failure. */
 bool device_alloc(void **ptr, size_t bytes, const char *what);
+/* Sum of a see pass's TAY_PAIR_SLOTS device pair counters (jit.cpp). */
+int read_pair_counters(const unsigned long long *counters, unsigned long long *pairs);
 void device_free(void *ptr);
 
 /* Sub-cell bits per axis of the storage order inside a cell (0: previous

@@ -516,8 +516,8 @@ TAY_API TayGpuProgram *tay_gpu_program_create(const char *const *sources, unsign
                   hip_ok(hipHostMalloc(&pass.context_host, ctx_bytes, hipHostMallocDefault), "hipHostMalloc") &&
                   hip_ok(hipMemsetAsync(pass.context, 0, ctx_bytes, stream()), "hipMemsetAsync");
         if (ok && passes[i].type == TAY_GPU_PASS_SEE)
-            ok = device_alloc((void **)&pass.pair_count, sizeof(unsigned long long), "pair counter") &&
-                 hip_ok(hipMemsetAsync(pass.pair_count, 0, sizeof(unsigned long long), stream()), "hipMemsetAsync");
+            ok = device_alloc((void **)&pass.pair_count, 8ull * TAY_PAIR_SLOTS, "pair counters") &&
+                 hip_ok(hipMemsetAsync(pass.pair_count, 0, 8ull * TAY_PAIR_SLOTS, stream()), "hipMemsetAsync");
         if (!ok) {
             tay_gpu_program_destroy(p);
             return nullptr;
@@ -668,6 +668,19 @@ TAY_API int tay_gpu_program_set_lane_walk(TayGpuProgram *p, unsigned width) {
     return 1;
 }
 
+/* The sum of a pass's TAY_PAIR_SLOTS pair counters. */
+int tay::read_pair_counters(const unsigned long long *counters, unsigned long long *pairs) {
+    unsigned long long host[TAY_PAIR_SLOTS];
+    hipStream_t s = stream();
+    if (!s || !hip_ok(hipMemcpyAsync(host, counters, sizeof(host), hipMemcpyDeviceToHost, s), "read pair count") ||
+        !hip_ok(hipStreamSynchronize(s), "read pair count"))
+        return 0;
+    *pairs = 0;
+    for (unsigned k = 0; k < TAY_PAIR_SLOTS; ++k)
+        *pairs += host[k];
+    return 1;
+}
+
 TAY_API int tay_gpu_program_pair_count(TayGpuProgram *p, unsigned pass_index, unsigned long long *pairs) {
     if (pass_index >= p->passes.size() || !p->passes[pass_index].pair_count) {
         set_error("pass %u is not a see pass", pass_index);
@@ -677,11 +690,7 @@ TAY_API int tay_gpu_program_pair_count(TayGpuProgram *p, unsigned pass_index, un
         set_error("program was created without pair counting");
         return 0;
     }
-    hipStream_t s = stream();
-    return s &&
-           hip_ok(hipMemcpyAsync(pairs, p->passes[pass_index].pair_count, sizeof(*pairs), hipMemcpyDeviceToHost, s),
-                  "read pair count") &&
-           hip_ok(hipStreamSynchronize(s), "read pair count");
+    return read_pair_counters(p->passes[pass_index].pair_count, pairs);
 }
 
 TAY_API int tay_gpu_program_reset_pair_counts(TayGpuProgram *p) {
@@ -689,7 +698,7 @@ TAY_API int tay_gpu_program_reset_pair_counts(TayGpuProgram *p) {
     if (!s)
         return 0;
     for (auto &pass : p->passes)
-        if (pass.pair_count && !hip_ok(hipMemsetAsync(pass.pair_count, 0, sizeof(unsigned long long), s), "reset"))
+        if (pass.pair_count && !hip_ok(hipMemsetAsync(pass.pair_count, 0, 8ull * TAY_PAIR_SLOTS, s), "reset"))
             return 0;
     return 1;
 }

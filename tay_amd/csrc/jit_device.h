@@ -893,10 +893,10 @@ __device__ __forceinline__ unsigned tay_see_tile(const TaySeeView &v, unsigned i
     return pairs;
 }
 
-__device__ __forceinline__ void tay_add_pairs(unsigned long long *counter, unsigned pairs) {
+__device__ __forceinline__ void tay_add_pairs(unsigned long long *counters, unsigned pairs) {
     const unsigned total = tay_wave_sum(pairs);
     if ((threadIdx.x & 63u) == 0u && total)
-        atomicAdd(counter, (unsigned long long)total);
+        atomicAdd(counters + blockIdx.x % TAY_PAIR_SLOTS, (unsigned long long)total);
 }
 
 /* Plain see kernel: wavefront k of the grid owns tile k. */
