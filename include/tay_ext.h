@@ -43,18 +43,20 @@ int tay_set_order_bits(TayState *state, TayGroup *group, int bits);
 
 /* Seers per lane group when a see tile starts (64, 32, 16 or 8; default
  * 64), as tay_gpu_program_set_tile_width.  In the reference see order
- * results do not depend on it; in the balanced order it is part of the
- * order in which each seer meets its pairs. */
+ * results do not depend on it; in the balanced order and in split passes
+ * (tay_set_see_split) it is part of the order in which each seer meets its
+ * pairs. */
 int tay_set_tile_width(TayState *state, unsigned width);
 
 /* Lane-walk width of the see kernels (0 to 64), as
- * tay_gpu_program_set_lane_walk; as the tile width, part of the balanced
- * order only. */
+ * tay_gpu_program_set_lane_walk; as the tile width, part of the order in
+ * the balanced order and in split passes only. */
 int tay_set_lane_walk(TayState *state, unsigned width);
 
 /* Seers per see tile (0 = by the seer count, the default; 64, 32 or 16),
- * as tay_gpu_program_set_tile_seers; as the tile width, part of the
- * balanced order only. */
+ * as tay_gpu_program_set_tile_seers; part of the order in the balanced
+ * order only (a split pass has 64 / lanes seers per tile whatever this
+ * says). */
 int tay_set_tile_seers(TayState *state, unsigned seers);
 
 /* Order in which a seer meets its pairs in see passes over a GPU Z-grid.

@@ -205,8 +205,9 @@ int tay_gpu_program_run_pass(TayGpuProgram *program, unsigned pass_index);
 /* Seers per lane group when a see tile starts: 64 (the default), 32, 16 or
  * 8.  A wavefront always owns 64 seers; smaller groups walk tighter cell
  * boxes one after another, which pays where agents are sparse.  Results do
- * not depend on it in the reference see order; in the balanced order it is
- * part of the order (see_order of TayGpuPassDesc).  No reference
+ * not depend on it in the reference see order; in the balanced order and
+ * in split passes it is part of the order (see_order and split_lanes of
+ * TayGpuPassDesc).  No reference
  * counterpart (a tuning knob of this library). */
 int tay_gpu_program_set_tile_width(TayGpuProgram *program, unsigned width);
 
@@ -215,14 +216,16 @@ int tay_gpu_program_set_tile_width(TayGpuProgram *program, unsigned width);
  * own cells, candidates one by one, all members at once) instead of
  * splitting into smaller groups swept one after another; 0 always splits.
  * Pays where agents are sparse.  As the tile width, part of the order only
- * in the balanced see order.  No reference counterpart. */
+ * in the balanced see order and in split passes.  No reference
+ * counterpart. */
 int tay_gpu_program_set_lane_walk(TayGpuProgram *program, unsigned width);
 
 /* Seers per see tile (one wavefront owns a tile): 64, 32 or 16, or 0 (the
  * default) to choose by the pass's seer count: 64 while that gives every
  * SIMD two wavefronts, fewer for smaller states, whose passes are bound by
- * the sweep of their few tiles.  As the tile width, part of the order only
- * in the balanced see order.  No reference counterpart. */
+ * the sweep of their few tiles.  Part of the order only in the balanced see
+ * order; a split pass always has 64 / split_lanes seers per tile.  No
+ * reference counterpart. */
 int tay_gpu_program_set_tile_seers(TayGpuProgram *program, unsigned seers);
 
 /* Pairs that passed the box test (agent_see calls) since the last reset;
