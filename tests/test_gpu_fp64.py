@@ -8,6 +8,8 @@ and every float sum must satisfy the condition-aware bound
     |x32 - x64| <= 1e-5 * sum |terms|
 (a plain relative bound is ill-conditioned where terms cancel, e.g. sums of
 offsets).  Sums of positive terms (SPH density) also meet 1e-5 relative.
+Each check runs the pass unsplit and split over 4 lanes per seer
+(tay_set_see_split), whose partial sums merge in a fixed tree.
 """
 import numpy as np
 import pytest
@@ -25,10 +27,13 @@ def see_only(model):
     return W.Model(model.dtype, [model.passes[0]], model.part, model.dims)
 
 
-def run_see(gpu, model, recs):
-    """Agents after one see pass, in storage order; a see pass changes only
-    the seers' sums, so positions and velocities are the inputs'."""
+def run_see(gpu, model, recs, lanes=0):
+    """Agents after one see pass (split over `lanes` lanes per seer when
+    lanes > 1), in storage order; a see pass changes only the seers' sums,
+    so positions and velocities are the inputs'."""
     with W.Simulation(gpu, see_only(model), recs, api.TAY_GPU_Z_GRID) as s:
+        if lanes > 1:
+            s.see_split(0, lanes)
         s.run(1)
         return s.agents()
 
@@ -59,10 +64,11 @@ def check_sum(x32, terms64, seer, n, name):
     return float((err / np.maximum(mag, 1e-30)).max())
 
 
-def test_plain_sums_within_fp64_bound(gpu):
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_plain_sums_within_fp64_bound(gpu, lanes):
     recs = W.plain_agents(20000, seed=41)
     r = 40.0
-    out = run_see(gpu, W.plain_model(r), recs)
+    out = run_see(gpu, W.plain_model(r), recs, lanes)
     p = out["p"][:, :3].astype(np.float32)
     i, j = candidate_pairs(p, r)
     i, j = i[box_ok(p, i, j, r)], j[box_ok(p, i, j, r)]
@@ -74,10 +80,11 @@ def test_plain_sums_within_fp64_bound(gpu):
     print("plain nsum worst relative to sum|terms|: %.3g" % worst)
 
 
-def test_reference_test_model_offsets_within_fp64_bound(gpu):
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_reference_test_model_offsets_within_fp64_bound(gpu, lanes):
     recs = W.test_agents(20000, seed=42, clump=0.1)
     r = 50.0
-    out = run_see(gpu, W.test_model(r), recs)
+    out = run_see(gpu, W.test_model(r), recs, lanes)
     p = out["p"][:, :3].astype(np.float32)
     i, j = candidate_pairs(p, r)
     ok = box_ok(p, i, j, r)
@@ -87,10 +94,11 @@ def test_reference_test_model_offsets_within_fp64_bound(gpu):
     print("test-model offsets worst relative to sum|terms|: %.3g" % worst)
 
 
-def test_flocking_sums_within_fp64_bound(gpu):
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_flocking_sums_within_fp64_bound(gpu, lanes):
     recs = W.flock_agents(50000, seed=43, clusters=16, sigma=30.0, box=1000.0)
     r = 25.0
-    out = run_see(gpu, W.flock_model(r), recs)
+    out = run_see(gpu, W.flock_model(r), recs, lanes)
     p = out["p"][:, :3].astype(np.float32)
     v = out["v"][:, :3].astype(np.float32)
     i, j = candidate_pairs(p, r)
@@ -107,10 +115,11 @@ def test_flocking_sums_within_fp64_bound(gpu):
     print("flocking worst relative to sum|terms|: alignment %.3g cohesion %.3g" % (w1, w2))
 
 
-def test_sph_density_within_fp64_relative(gpu):
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_sph_density_within_fp64_relative(gpu, lanes):
     recs, lo, hi = W.sph_particles(100000, h=1.0)
     ctx = W.sph_context(h=1.0, dt=1e-3, box_min=lo, box_max=hi)
-    out = run_see(gpu, W.sph_model(ctx, 1.0), recs)
+    out = run_see(gpu, W.sph_model(ctx, 1.0), recs, lanes)
     p = out["p"][:, :3].astype(np.float32)
     h = 1.0
     i, j = candidate_pairs(p, h)
@@ -131,7 +140,8 @@ def test_sph_density_within_fp64_relative(gpu):
     assert err.max() <= TOL
 
 
-def test_full_size_flocking_positions_counts_and_offsets(gpu):
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_full_size_flocking_positions_counts_and_offsets(gpu, lanes):
     """At the full 1M-agent flocking configuration (64 Gaussian clusters,
     sigma 30, box 2000, radius 25, cell size 25) with the reference's test
     model (b_buffer += p_b - p_a, count += 1; test/agent.c:2-8), one see
@@ -146,6 +156,9 @@ def test_full_size_flocking_positions_counts_and_offsets(gpu):
     r = 25.0
     with W.Simulation(gpu, see_only(W.test_model(r, part=25.0)), recs, api.TAY_GPU_Z_GRID) as s:
         gpu.tay_set_pair_counting(s.state, 1)
+        if lanes > 1:
+            s.see_split(0, lanes)
+            gpu.tay_set_lane_walk(s.state, 0)
         s.run(1)
         out = s.agents()
         pairs = gpu.pair_counts(s.state, 0)[1]
