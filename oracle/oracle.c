@@ -1364,8 +1364,15 @@ static void gpu_z_grid_see_tiles(TayPass *p, unsigned beg, unsigned end, unsigne
     /* a split pass: partial records 1..K-1 of the tile's seers */
     const unsigned K = p->split_lanes > 1u ? p->split_lanes : 1u;
     const size_t asize = (size_t)sg->agent_size;
-    char *parts = K > 1u ? (char *)malloc(asize * S * (K - 1u)) : NULL;
-    char **rec = K > 1u ? (char **)malloc(sizeof(char *) * S * K) : NULL;
+    char *rec[64];          /* a split pass has S * K = 64 partials per tile */
+    char *parts = NULL;
+    if (K > 1u) {
+        parts = (char *)malloc(asize * S * (K - 1u));
+        if (!parts) {
+            fprintf(stderr, "oracle: no memory for the split pass's partial records\n");
+            abort();
+        }
+    }
     for (unsigned i0 = beg; i0 < end; i0 += S) {
         const unsigned n_active = seers - i0 < S ? seers - i0 : S;
         if (K > 1u)
@@ -1426,7 +1433,6 @@ static void gpu_z_grid_see_tiles(TayPass *p, unsigned beg, unsigned end, unsigne
                         p->split_combine(rec[m * K + k], rec[m * K + k + step]);
     }
     free(parts);
-    free(rec);
 }
 
 /* -------------------------------------------------------------- see/act */
