@@ -284,7 +284,17 @@ struct TaySeeView {
  * fewer compact rounds carry the same pairs, while runs keep the loads of
  * a round on few cache lines.  Each seer meets the same pairs in that
  * fixed order instead of the sweep order; TAY_BALANCE_RUN 64 (or M <= 64)
- * is the sweep order. */
+ * is the sweep order.
+ *
+ * SPLIT (a split see pass, 2, 4 or 8): the tile holds 64 / SPLIT seers and
+ * lane s * SPLIT + part works for seer s: part 0 on the seer's record, the
+ * others on partial copies emptied by `clear`.  A swept group fills its
+ * chunks 64 candidates at a time in sweep order and lane `part` takes
+ * slots part, part + SPLIT, ..., so the candidate at sweep position pos
+ * goes to partial pos mod SPLIT; a walking group hands each seer's cell
+ * rows (x, y) to its lanes by row index mod SPLIT.  At the end of the tile
+ * the partials merge into part 0 by `combine` over lane shuffles, in a
+ * fixed tree (0 + 1), (2 + 3), ..., then pairs of those. */
 template <class T, unsigned SEER_WORDS, unsigned SEEN_WORDS, int NARROW, bool SKIP_SELF, bool SEEN_GRID, int DIMS,
           bool COUNT, unsigned CTX_WORDS, bool ROW_WALK, bool BALANCED, unsigned SPLIT, class Call, class Clear,
           class Combine>
